@@ -1,0 +1,150 @@
+/* mxc_gpu.h - C ABI of the MI355X (gfx950) device layer of the H^2 solve path.
+ *
+ * This is the boundary that replaces the reference's CUDA device layer
+ * (delta-leader/mxc): gpu_handles.cuh:20-25, device_factorize.cuh:67-78,
+ * device_csr_matrix.cuh:65-77 and the device GMRES. Plain C: opaque handles,
+ * pointers and sizes, int status codes (0 = success), no C++/STL/MPI/complex
+ * types. Arithmetic is real (the reference's complex<double> has zero
+ * imaginary parts for these kernels). Device pointers are marked d_.
+ *
+ * One process drives one GPU; the reference's MPI/NCCL distribution collapses
+ * to a single rank holding the whole tree, so there are no communicators.
+ */
+#ifndef MXC_GPU_H
+#define MXC_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+enum {
+  MXG_OK = 0,
+  MXG_ERR_ARG = 1,      /* invalid argument or size beyond a documented limit */
+  MXG_ERR_HIP = 2,      /* a HIP runtime call failed */
+  MXG_ERR_NOMEM = 3,    /* device allocation failed */
+  MXG_ERR_STATE = 4,    /* call out of order (e.g. level not constructed) */
+  MXG_ERR_SINGULAR = 5  /* factorization flagged a singular diagonal block */
+};
+
+/* kernel functions K(d) with K(0) = diag (kernel.hpp:28-61) */
+enum { MXG_KERNEL_LAPLACE = 0, MXG_KERNEL_YUKAWA = 1, MXG_KERNEL_GAUSSIAN = 2 };
+typedef struct {
+  int id;        /* MXG_KERNEL_* */
+  double diag;   /* K(0) */
+  double alpha;  /* Yukawa decay / Gaussian width */
+} mxg_kernel_t;
+
+/* precision of a factorization or of an operator application */
+enum { MXG_FP64 = 0, MXG_FP32 = 1 };
+
+typedef struct mxg_ctx_s* mxg_ctx_t; /* device + stream (replaces deviceHandle_t) */
+typedef struct mxg_h2_s* mxg_h2_t;   /* one H^2 matrix resident on the device   */
+typedef struct mxg_ulv_s* mxg_ulv_t; /* one ULV factorization of an H^2 matrix  */
+
+/* ---- context: replaces cudaSetDevice/initGpuEnvs/finalizeGpuEnvs
+ *      (gpu_handles.cu:5-12, 54-71) ---- */
+int mxg_init(int device, mxg_ctx_t* ctx);
+int mxg_finalize(mxg_ctx_t ctx);
+int mxg_device_count(int* count);
+int mxg_synchronize(mxg_ctx_t ctx);
+const char* mxg_last_error(void);
+/* HIP stream of the context (a hipStream_t) */
+void* mxg_stream(mxg_ctx_t ctx);
+
+/* device memory for callers without an allocator of their own */
+int mxg_malloc(mxg_ctx_t ctx, size_t bytes, void** d_ptr);
+int mxg_free(mxg_ctx_t ctx, void* d_ptr);
+int mxg_memcpy_h2d(mxg_ctx_t ctx, void* d_dst, const void* src, size_t bytes);
+int mxg_memcpy_d2h(mxg_ctx_t ctx, void* dst, const void* d_src, size_t bytes);
+int mxg_memcpy_d2d(mxg_ctx_t ctx, void* d_dst, const void* d_src, size_t bytes);
+int mxg_memset(mxg_ctx_t ctx, void* d_dst, int value, size_t bytes);
+
+/* C = alpha op(A) op(B) + beta C on device matrices (column major; ta/tb != 0
+ * transpose), on the matrix cores; precision MXG_FP64 (double*) or MXG_FP32 (float*). */
+int mxg_gemm(mxg_ctx_t ctx, int precision, int ta, int tb, long long m, long long n, long long k, double alpha,
+             const void* d_A, long long lda, const void* d_B, long long ldb, double beta, void* d_C, long long ldc);
+
+/* ---- kernel blocks: replaces gen_matrix (kernel.cpp:71-85) on the device.
+ * A[i + j*lda] = K(|bi_i - bj_j|), column major, points 3 doubles each. ---- */
+int mxg_gen_block(mxg_ctx_t ctx, mxg_kernel_t k, long long m, long long n, const double* d_bi,
+                  const double* d_bj, double* d_A, long long lda);
+
+/* Dense FP64 check y = A x, matrix free (main.cpp:83-84,114 with kernel.cpp:25-35). */
+int mxg_dense_matvec(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n, const double* d_x,
+                     double* d_y);
+
+/* ---- H^2 matrix: replaces createSpMatrixDesc/createDeviceCsr/createMatrixDesc and
+ *      the numeric work of H2Matrix::construct (h2matrix.cpp:72-249). ---- */
+enum {
+  MXG_H2_FACTORABLE = 1 /* keep the assembled upper-level near blocks for mxg_ulv_create */
+};
+/* bodies: host array, tree order (3 doubles per body). */
+int mxg_h2_create(mxg_ctx_t ctx, mxg_kernel_t k, const double* bodies, long long nbodies, long long levels,
+                  int flags, mxg_h2_t* h);
+int mxg_h2_destroy(mxg_h2_t h);
+
+/* Construct one level, leaves first (level = levels) then upward to 0.
+ * dims: box sizes (leaf: body counts; upper: sum of the two children's ranks,
+ *   which must equal what the lower level returned).
+ * leaf_begin: first body of each leaf box (leaf level only, else NULL).
+ * near_rp/near_col, far_rp/far_col: level-local CSR lists (columns are box
+ *   indices within the level, rows sorted).
+ * far_off[nboxes+1], far_idx: sampled far-body indices per box (may be empty).
+ * Basis rule (h2matrix.cpp:13-63): rank = Eigen CPQR threshold rank when
+ *   tol > 0 (else min(M, #far)), capped at maxrank when maxrank > 0; orth != 0
+ *   builds the orthogonal basis, orth == 0 the interpolative one.
+ * ranks_out[nboxes]: receives the skeleton ranks. */
+int mxg_h2_construct_level(mxg_h2_t h, long long level, long long nboxes, const long long* dims,
+                           const long long* leaf_begin, const long long* near_rp, const long long* near_col,
+                           const long long* far_rp, const long long* far_col, const long long* far_off,
+                           const long long* far_idx, double tol, long long maxrank, int orth,
+                           long long* ranks_out);
+
+/* Copy of level data to the host (copyDataOutMatrixDesc, device_precondition.cu:204-208).
+ * what: 0 = near blocks A, 1 = bases Q (M*M per box), 2 = R (M*M per box), 3 = couplings C,
+ *       4 = skeleton body indices (as doubles). count = number of doubles. */
+int mxg_h2_download(mxg_h2_t h, long long level, int what, double* dst, long long count);
+long long mxg_h2_level_size(mxg_h2_t h, long long level, int what);
+
+/* H^2 matvec y = A x on the device (matVecDeviceH2, device_csr_matrix.cu:328-338).
+ * x, y: tree-ordered FP64 vectors of length nbodies, nrhs columns with leading
+ * dimension ld. precision MXG_FP32 applies an FP32 copy of the operator data
+ * (built on first use) with FP32 inputs rounded from x and FP64 outputs. */
+int mxg_h2_matvec(mxg_h2_t h, const double* d_x, double* d_y, long long nrhs, long long ld, int precision);
+
+/* ---- ULV factorization: replaces compute_factorize / factorizeCopyNext / check_info
+ *      (device_factorize.cu:56-142, h2matrix.cpp:346-446, device_precondition.cu:210-215) ---- */
+int mxg_ulv_create(mxg_h2_t h, int precision, mxg_ulv_t* f);
+int mxg_ulv_destroy(mxg_ulv_t f);
+/* info_per_level[levels+1]: number of boxes whose rr block had |det| <= DBL_MIN. */
+int mxg_ulv_info(mxg_ulv_t f, int* info_per_level);
+/* x <- M^{-1} x in place (matSolvePreconditionDeviceH2, device_factorize.cu:244-254). */
+int mxg_ulv_solve(mxg_ulv_t f, double* d_x, long long nrhs, long long ld);
+/* seconds spent in the last factorization, measured with HIP events */
+double mxg_ulv_factor_seconds(mxg_ulv_t f);
+
+/* ---- GMRES: replaces solveDeviceGMRES (device_gmres.cu:26-115) ---- */
+enum {
+  MXG_GMRES_REFERENCE = 0, /* CGS1 Arnoldi, fixed inner iterations (solver.cpp:144-201) */
+  MXG_GMRES_CGS2 = 1       /* CGS2 Arnoldi, Givens updates, inner exit on the estimate */
+};
+typedef struct {
+  double tol;          /* stop when ||b - A x|| / ||b|| < tol */
+  int restart;         /* inner iterations per cycle */
+  int max_cycles;      /* outer cycles */
+  int mode;            /* MXG_GMRES_* */
+  int inner_precision; /* MXG_FP64, or MXG_FP32 for FP32 inner operator applications */
+} mxg_gmres_opts_t;
+/* Solves A x = b with left preconditioner M. x holds the initial guess on entry.
+ * resid[0..max_cycles]: relative residual (H^2 operator) after each cycle.
+ * iters_out: cycles run; inner_out: total inner iterations. */
+int mxg_gmres(mxg_h2_t A, mxg_ulv_t M, const mxg_gmres_opts_t* opts, const double* d_b, double* d_x,
+              double* resid, long long* iters_out, long long* inner_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MXC_GPU_H */

@@ -1,0 +1,145 @@
+// Shared device-layer utilities: error handling, device buffers, wave
+// reductions, kernel-function evaluation. gfx950 (CDNA4): 64-lane wavefronts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mxc_gpu.h"
+
+namespace mxg {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define MXG_HIP(x)                                                                              \
+  do {                                                                                          \
+    hipError_t e_ = (x);                                                                        \
+    if (e_ != hipSuccess)                                                                       \
+      throw ::mxg::Error(MXG_ERR_HIP, std::string(#x) + " failed: " + hipGetErrorString(e_));   \
+  } while (0)
+
+#define MXG_REQUIRE(cond, msg)                                   \
+  do {                                                           \
+    if (!(cond)) throw ::mxg::Error(MXG_ERR_ARG, (msg));         \
+  } while (0)
+
+// Run a C-ABI body, converting exceptions into status codes + last error.
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return MXG_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return MXG_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return MXG_ERR_ARG;
+  }
+}
+
+// Owning device buffer.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error(MXG_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+    }
+    n = count;
+  }
+  void alloc_zero(size_t count, hipStream_t s) {
+    alloc(count);
+    if (n) MXG_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+  void upload(const T* src, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count) MXG_HIP(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+  // grow-only scratch
+  void reserve(size_t count) {
+    if (count > n) alloc(count);
+  }
+};
+
+inline unsigned ceil_div(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+// ---- device helpers ----
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Kernel function K(d) from the squared distance (kernel.hpp:28-61).
+// d = sqrt(d2) and 1/d are correctly rounded IEEE operations; exp is the
+// device libm exp (within a few ulp of the host's).
+__device__ inline double kernel_eval(int id, double diag, double alpha, double d2) {
+  if (id == MXG_KERNEL_GAUSSIAN) return d2 == 0. ? 1.0001 : exp(-alpha * d2);
+  if (d2 == 0.) return diag;
+  double d = sqrt(d2);
+  if (id == MXG_KERNEL_YUKAWA) return exp(-alpha * d) / d;
+  return 1. / d;
+}
+
+__device__ inline double dist2(const double* a, const double* b) {
+  double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Binary search: largest t with prefix[t] <= x (prefix non-decreasing, prefix[0] = 0).
+__device__ inline int find_task(const long long* prefix, int ntasks, long long x) {
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= x) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+}  // namespace mxg
+
+// Context object (opaque in the C ABI).
+struct mxg_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};

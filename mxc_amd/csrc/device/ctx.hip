@@ -1,0 +1,122 @@
+// Context, memory helpers and error reporting of the device layer.
+// Replaces gpu_handles.cu:5-12 (device selection) and :54-71 (streams and
+// handles); there are no BLAS/SPARSE handles: every kernel is our own.
+#include "common.hpp"
+
+namespace mxg {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace mxg
+
+using namespace mxg;
+
+extern "C" {
+
+const char* mxg_last_error(void) { return g_last_error.c_str(); }
+
+int mxg_device_count(int* count) {
+  return guarded([&] {
+    MXG_REQUIRE(count, "count is null");
+    MXG_HIP(hipGetDeviceCount(count));
+  });
+}
+
+int mxg_init(int device, mxg_ctx_t* ctx) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    int n = 0;
+    MXG_HIP(hipGetDeviceCount(&n));
+    MXG_REQUIRE(n > 0, "no HIP device visible");
+    MXG_REQUIRE(device >= 0 && device < n, "device index out of range");
+    MXG_HIP(hipSetDevice(device));
+    auto* c = new mxg_ctx_s();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      throw Error(MXG_ERR_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+    }
+    *ctx = c;
+  });
+}
+
+int mxg_finalize(mxg_ctx_t ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    MXG_HIP(hipSetDevice(ctx->device));
+    MXG_HIP(hipStreamSynchronize(ctx->stream));
+    MXG_HIP(hipStreamDestroy(ctx->stream));
+    delete ctx;
+  });
+}
+
+void* mxg_stream(mxg_ctx_t ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int mxg_synchronize(mxg_ctx_t ctx) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    MXG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mxg_malloc(mxg_ctx_t ctx, size_t bytes, void** d_ptr) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx && d_ptr, "null argument");
+    MXG_HIP(hipSetDevice(ctx->device));
+    *d_ptr = nullptr;
+    if (bytes == 0) return;
+    hipError_t e = hipMalloc(d_ptr, bytes);
+    if (e != hipSuccess) throw Error(MXG_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  });
+}
+
+int mxg_free(mxg_ctx_t ctx, void* d_ptr) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    if (d_ptr) MXG_HIP(hipFree(d_ptr));
+  });
+}
+
+int mxg_memcpy_h2d(mxg_ctx_t ctx, void* d_dst, const void* src, size_t bytes) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    if (bytes == 0) return;
+    MXG_REQUIRE(d_dst && src, "null pointer");
+    MXG_HIP(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    MXG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mxg_memcpy_d2h(mxg_ctx_t ctx, void* dst, const void* d_src, size_t bytes) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    if (bytes == 0) return;
+    MXG_REQUIRE(dst && d_src, "null pointer");
+    MXG_HIP(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    MXG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mxg_memcpy_d2d(mxg_ctx_t ctx, void* d_dst, const void* d_src, size_t bytes) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    if (bytes == 0) return;
+    MXG_REQUIRE(d_dst && d_src, "null pointer");
+    MXG_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    MXG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mxg_memset(mxg_ctx_t ctx, void* d_dst, int value, size_t bytes) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    if (bytes == 0) return;
+    MXG_REQUIRE(d_dst, "null pointer");
+    MXG_HIP(hipMemsetAsync(d_dst, value, bytes, ctx->stream));
+    MXG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+}  // extern "C"

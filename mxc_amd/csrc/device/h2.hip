@@ -1,0 +1,778 @@
+// H^2 matrix on the device: level construction and the matvec passes.
+//
+// Construction of one level follows H2Matrix::construct (h2matrix.cpp:72-249)
+// at a single rank:
+//   leaf:  skeleton = the box's bodies, Q_in = I, near blocks generated;
+//   upper: skeleton = the children's skeleton prefixes, Q_in = blockdiag of the
+//          children's R, and (for a factorable matrix) each parent near block
+//          receives the couplings of its far child pairs (h2matrix.cpp:204-233);
+//   basis (compute_basis, h2matrix.cpp:13-63) for every box at once:
+//          G = K(far sample, skeleton) -> Householder R -> Eigen-rule CPQR ->
+//          rank, T = R11^{-1} R12, skeleton permutation ->
+//          orthogonal: QR of triu(Q_in) P [I; T^T] gives Q (M x M) and R;
+//          interpolative: Q = [P [I; T^T], 0], R = I;
+//   couplings (h2matrix.cpp:222-242): C = triu(R_y) K(S_y, S_x) triu(R_x)^T
+//          (orthogonal) or K(S_y, S_x) (interpolative).
+// The matvec (h2matrix.cpp:251-344) runs as one kernel per level and pass,
+// one workgroup per box.
+#include "h2.hpp"
+
+#include <algorithm>
+#include <numeric>
+
+namespace mxg {
+namespace {
+
+// ---------------- small batched kernels ----------------
+
+template <class T>
+struct Copy2d {
+  const T* src;
+  T* dst;
+  int m, n, lds, ldd;
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) k_copy2d(const Copy2d<T>* __restrict__ tasks, const long long* __restrict__ prefix,
+                                                int ntasks) {
+  const long long blk = blockIdx.x;
+  const int t = find_task(prefix, ntasks, blk);
+  const Copy2d<T> c = tasks[t];
+  const long long e = (blk - prefix[t]) * 256 + threadIdx.x;
+  if (e >= (long long)c.m * c.n) return;
+  const int i = (int)(e % c.m), j = (int)(e / c.m);
+  c.dst[i + (long long)j * c.ldd] = c.src[i + (long long)j * c.lds];
+}
+
+template <class Td, class Ts>
+__global__ void k_convert(Td* __restrict__ dst, const Ts* __restrict__ src, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = (Td)src[i];
+}
+
+__global__ void k_leaf_init(int nb, const long long* __restrict__ offX, const long long* __restrict__ offQ,
+                            const int* __restrict__ dims, const long long* __restrict__ begin, int* __restrict__ S,
+                            double* __restrict__ Q) {
+  const int i = blockIdx.x;
+  const int M = dims[i];
+  for (int t = threadIdx.x; t < M; t += blockDim.x) {
+    S[offX[i] + t] = (int)(begin[i] + t);
+    Q[offQ[i] + t + (long long)t * M] = 1.;
+  }
+}
+
+// Interpolative decomposition post-processing of one box (h2matrix.cpp:92-116).
+struct PostTask {
+  const double* R0;  // CPQR output, K x M (ld ldr)
+  const int* perm;   // M
+  double* Q;         // M x M (holds Q_in on entry)
+  double* R;         // M x M
+  int* S;            // M skeleton indices
+  int* Stmp;         // M scratch
+  double* T;         // r x (M - r) scratch
+  double* W;         // M x r: P [I; T^T]
+  int M, r, ldr, mode;  // mode 1: rank 0 or M; mode 2: 0 < r < M interpolative; mode 3: same, orthogonal
+};
+
+__global__ void __launch_bounds__(256) k_post(const PostTask* __restrict__ tasks) {
+  const PostTask t = tasks[blockIdx.x];
+  const int M = t.M, r = t.r;
+  if (t.mode == 1) {
+    // R = triu(Q_in), Q = I
+    for (long long e = threadIdx.x; e < (long long)M * M; e += 256) {
+      const int i = (int)(e % M), j = (int)(e / M);
+      t.R[e] = (i <= j) ? t.Q[e] : 0.;
+    }
+    __syncthreads();
+    for (long long e = threadIdx.x; e < (long long)M * M; e += 256) t.Q[e] = ((e % M) == (e / M)) ? 1. : 0.;
+    return;
+  }
+  // T = R11^{-1} R12, one column per thread (back substitution)
+  for (int c = threadIdx.x; c < M - r; c += 256) {
+    double* x = t.T + (long long)c * r;
+    const double* b = t.R0 + (long long)(r + c) * t.ldr;
+    for (int i = r - 1; i >= 0; i--) {
+      double s = b[i];
+      for (int j = i + 1; j < r; j++) s -= t.R0[i + (long long)j * t.ldr] * x[j];
+      x[i] = s / t.R0[i + (long long)i * t.ldr];
+    }
+  }
+  for (int j = threadIdx.x; j < M; j += 256) t.Stmp[j] = t.S[t.perm[j]];
+  __syncthreads();
+  for (int j = threadIdx.x; j < M; j += 256) t.S[j] = t.Stmp[j];
+  // W = P [I; T^T]: row perm[j] of W is row j of [I; T^T]
+  for (long long e = threadIdx.x; e < (long long)M * r; e += 256) {
+    const int j = (int)(e % M), c = (int)(e / M);
+    const double v = (j < r) ? (j == c ? 1. : 0.) : t.T[c + (long long)(j - r) * r];
+    t.W[t.perm[j] + (long long)c * M] = v;
+  }
+  if (t.mode == 2) {
+    __syncthreads();
+    for (long long e = threadIdx.x; e < (long long)M * M; e += 256) {
+      const int i = (int)(e % M), j = (int)(e / M);
+      t.Q[e] = (j < r) ? t.W[i + (long long)j * M] : 0.;
+      t.R[e] = (i == j && j < r) ? 1. : 0.;
+    }
+  }
+}
+
+// ---------------- matvec kernels (one workgroup per box) ----------------
+
+template <class T>
+__global__ void __launch_bounds__(256) k_up(const int* __restrict__ dims, const int* __restrict__ ranks,
+                                            const long long* __restrict__ offQ, const long long* __restrict__ offX,
+                                            const long long* __restrict__ offZ, const T* __restrict__ Q,
+                                            const T* __restrict__ x, T* __restrict__ z) {
+  const int i = blockIdx.x;
+  const int M = dims[i], k = ranks[i];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const T* Qi = Q + offQ[i];
+  const T* xi = x + offX[i];
+  for (int c = wave; c < k; c += 4) {
+    const T* q = Qi + (long long)c * M;
+    T s = 0;
+    for (int r = lane; r < M; r += 64) s += q[r] * xi[r];
+    s = wave_sum(s);
+    if (lane == 0) z[offZ[i] + c] = s;
+  }
+}
+
+// w_i (+)= sum_far C_ij z_j into LDS, then y_i = Q_i[:, :k] w_i (+ near field).
+template <class T, bool LEAF>
+__global__ void __launch_bounds__(256) k_down(const int* __restrict__ dims, const int* __restrict__ ranks,
+                                              const long long* __restrict__ offQ, const long long* __restrict__ offX,
+                                              const long long* __restrict__ offZ, const long long* __restrict__ frp,
+                                              const int* __restrict__ fcol, const long long* __restrict__ offC,
+                                              const long long* __restrict__ nrp, const int* __restrict__ ncol,
+                                              const long long* __restrict__ offA, const T* __restrict__ Q,
+                                              const T* __restrict__ C, const T* __restrict__ A, const T* __restrict__ z,
+                                              const T* __restrict__ w_in, const T* __restrict__ x, T* __restrict__ y) {
+  extern __shared__ unsigned char smem[];
+  T* sw = reinterpret_cast<T*>(smem);
+  const int i = blockIdx.x;
+  const int M = dims[i], k = ranks[i];
+  for (int r = threadIdx.x; r < k; r += 256) {
+    T s = w_in[offZ[i] + r];
+    for (long long ij = frp[i]; ij < frp[i + 1]; ij++) {
+      const int j = fcol[ij];
+      const int kj = ranks[j];
+      const T* c = C + offC[ij] + r;
+      const T* zj = z + offZ[j];
+      for (int cc = 0; cc < kj; cc++) s += c[(long long)cc * k] * zj[cc];
+    }
+    sw[r] = s;
+  }
+  __syncthreads();
+  const T* Qi = Q + offQ[i];
+  for (int r = threadIdx.x; r < M; r += 256) {
+    T s = 0;
+    for (int c = 0; c < k; c++) s += Qi[r + (long long)c * M] * sw[c];
+    if (LEAF) {
+      for (long long ij = nrp[i]; ij < nrp[i + 1]; ij++) {
+        const int j = ncol[ij];
+        const int Mj = dims[j];
+        const T* a = A + offA[ij] + r;
+        const T* xj = x + offX[j];
+        for (int c = 0; c < Mj; c++) s += a[(long long)c * M] * xj[c];
+      }
+    }
+    y[offX[i] + r] = s;
+  }
+}
+
+long long lookup(const std::vector<long long>& rp, const std::vector<long long>& col, long long i, long long j) {
+  if (i < 0 || (long long)rp.size() <= i + 1) return -1;
+  auto b = col.begin() + rp[i], e = col.begin() + rp[i + 1];
+  auto it = std::lower_bound(b, e, j);
+  return (it != e && *it == j) ? (long long)(it - col.begin()) : -1;
+}
+
+template <class T>
+std::vector<T> to_vec(const long long* p, long long n) {
+  std::vector<T> v(n > 0 ? n : 0);
+  for (long long i = 0; i < n; i++) v[i] = (T)p[i];
+  return v;
+}
+
+}  // namespace
+
+template <class T>
+void copy2d_batched(const std::vector<const T*>& src, const std::vector<T*>& dst, const std::vector<int>& m,
+                    const std::vector<int>& n, const std::vector<int>& lds, const std::vector<int>& ldd, hipStream_t s) {
+  std::vector<Copy2d<T>> tasks;
+  std::vector<long long> prefix{0};
+  for (size_t t = 0; t < src.size(); t++) {
+    long long cnt = (long long)m[t] * n[t];
+    if (cnt <= 0) continue;
+    tasks.push_back(Copy2d<T>{src[t], dst[t], m[t], n[t], lds[t], ldd[t]});
+    prefix.push_back(prefix.back() + (cnt + 255) / 256);
+  }
+  if (tasks.empty()) return;
+  MXG_REQUIRE(prefix.back() < (1ll << 31), "copy grid too large");
+  DevBuf<Copy2d<T>> dt;
+  DevBuf<long long> dp;
+  dt.upload(tasks, s);
+  dp.upload(prefix, s);
+  hipLaunchKernelGGL(k_copy2d<T>, dim3((unsigned)prefix.back()), dim3(256), 0, s, dt.p, dp.p, (int)tasks.size());
+  MXG_HIP(hipGetLastError());
+  MXG_HIP(hipStreamSynchronize(s));
+}
+
+template void copy2d_batched<double>(const std::vector<const double*>&, const std::vector<double*>&, const std::vector<int>&,
+                                     const std::vector<int>&, const std::vector<int>&, const std::vector<int>&, hipStream_t);
+template void copy2d_batched<float>(const std::vector<const float*>&, const std::vector<float*>&, const std::vector<int>&,
+                                    const std::vector<int>&, const std::vector<int>&, const std::vector<int>&, hipStream_t);
+template void copy2d_batched<int>(const std::vector<const int*>&, const std::vector<int*>&, const std::vector<int>&,
+                                  const std::vector<int>&, const std::vector<int>&, const std::vector<int>&, hipStream_t);
+
+template <class Tdst, class Tsrc>
+void convert(Tdst* dst, const Tsrc* src, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL((k_convert<Tdst, Tsrc>), dim3(blocks), dim3(256), 0, s, dst, src, n);
+  MXG_HIP(hipGetLastError());
+}
+template void convert<float, double>(float*, const double*, size_t, hipStream_t);
+template void convert<double, float>(double*, const float*, size_t, hipStream_t);
+template void convert<double, double>(double*, const double*, size_t, hipStream_t);
+
+namespace {
+
+// Budget of doubles for the far-sample matrices G of one construction chunk.
+constexpr size_t G_BUDGET = size_t(1) << 28;
+
+void construct_basis(mxg_h2_s* h, Level& Lv, long long l, const long long* far_off, const long long* far_idx, double tol,
+                     long long maxrank, int orth) {
+  hipStream_t s = h->ctx->stream;
+  const long long nb = Lv.nb;
+  std::vector<long long> nf(nb, 0);
+  if (l > 0 && far_off)
+    for (long long i = 0; i < nb; i++) nf[i] = far_off[i + 1] - far_off[i];
+  long long nfar = (l > 0 && far_off) ? far_off[nb] : 0;
+  DevBuf<int> d_far;
+  if (nfar > 0) {
+    std::vector<int> fi(nfar);
+    for (long long t = 0; t < nfar; t++) {
+      MXG_REQUIRE(far_idx[t] >= 0 && far_idx[t] < h->n, "far body index out of range");
+      fi[t] = (int)far_idx[t];
+    }
+    d_far.upload(fi, s);
+  }
+  DevBuf<int> d_perm, d_rank, d_stmp;
+  d_perm.alloc(std::max<long long>(Lv.lenX, 1));
+  d_stmp.alloc(std::max<long long>(Lv.lenX, 1));
+  d_rank.alloc_zero(nb, s);
+  std::fill(Lv.ranks.begin(), Lv.ranks.end(), 0);
+
+  long long b0 = 0;
+  while (b0 < nb) {
+    // chunk [b0, b1) of boxes whose G matrices fit the budget
+    long long b1 = b0;
+    size_t gsz = 0;
+    std::vector<size_t> goff;
+    while (b1 < nb) {
+      long long M = Lv.dims[b1];
+      long long K = std::min(M, nf[b1]);
+      size_t need = K > 0 ? (size_t)nf[b1] * M : 0;
+      if (b1 > b0 && gsz + need > G_BUDGET) break;
+      goff.push_back(gsz);
+      gsz += need;
+      b1++;
+    }
+    DevBuf<double> G, tau, work;
+    G.alloc(std::max<size_t>(gsz, 1));
+    size_t tsz = 0, wsz = 0;
+    for (long long i = b0; i < b1; i++) {
+      tsz += (size_t)std::min(Lv.dims[i], nf[i]);
+      wsz += 3 * (size_t)Lv.dims[i];
+    }
+    tau.alloc(std::max<size_t>(tsz, 1));
+    work.alloc(std::max<size_t>(wsz, 1));
+
+    std::vector<GenTask> gens;
+    std::vector<QrTask> qrs;
+    std::vector<CpqrTask> cps;
+    size_t toff = 0, woff = 0;
+    for (long long i = b0; i < b1; i++) {
+      const long long M = Lv.dims[i], K = std::min(M, nf[i]);
+      if (K <= 0) continue;
+      double* Gi = G.p + goff[i - b0];
+      gens.push_back(GenTask{d_far.p + far_off[i], Lv.S.p + Lv.offX[i], 0, 0, Gi, (int)nf[i], (int)M, (int)nf[i], 0});
+      if (K < nf[i]) qrs.push_back(QrTask{Gi, tau.p + toff, (int)nf[i], (int)M, (int)nf[i], 1});
+      toff += K;
+      long long steps = maxrank > 0 ? std::min(K, maxrank) : K;
+      cps.push_back(CpqrTask{Gi, d_perm.p + Lv.offX[i], d_rank.p + i, work.p + woff, (int)K, (int)M, (int)nf[i], (int)steps,
+                             tol, (int)maxrank, 0});
+      woff += 3 * M;
+    }
+    gen_tasks<double>(gens, h->bodies.p, h->k, s);
+    qr_batched(qrs, s);
+    cpqr_batched(cps, s);
+    std::vector<int> rk(nb);
+    MXG_HIP(hipMemcpyAsync(rk.data(), d_rank.p, nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    MXG_HIP(hipStreamSynchronize(s));
+
+    // post-processing (h2matrix.cpp:92-116)
+    std::vector<PostTask> posts;
+    std::vector<long long> orth_boxes;
+    size_t tneed = 0, wneed = 0;
+    for (long long i = b0; i < b1; i++) {
+      const long long M = Lv.dims[i], K = std::min(M, nf[i]);
+      if (K <= 0) continue;
+      const long long r = rk[i];
+      Lv.ranks[i] = r;
+      if (0 < r && r < M) {
+        tneed += (size_t)r * (M - r);
+        wneed += (size_t)M * r;
+      }
+    }
+    DevBuf<double> Tm, Wm, P, ptau;
+    Tm.alloc(std::max<size_t>(tneed, 1));
+    Wm.alloc_zero(std::max<size_t>(wneed, 1), s);
+    std::vector<size_t> wof(nb, 0);
+    size_t to = 0, wo = 0;
+    for (long long i = b0; i < b1; i++) {
+      const long long M = Lv.dims[i], K = std::min(M, nf[i]);
+      if (K <= 0) continue;
+      const long long r = Lv.ranks[i];
+      PostTask pt{};
+      pt.Q = Lv.Q.p + Lv.offQ[i];
+      pt.R = Lv.R.p + Lv.offQ[i];
+      pt.M = (int)M;
+      pt.r = (int)r;
+      if (0 < r && r < M) {
+        pt.R0 = G.p + goff[i - b0];
+        pt.ldr = (int)nf[i];
+        pt.perm = d_perm.p + Lv.offX[i];
+        pt.S = Lv.S.p + Lv.offX[i];
+        pt.Stmp = d_stmp.p + Lv.offX[i];
+        pt.T = Tm.p + to;
+        pt.W = Wm.p + wo;
+        pt.mode = orth ? 3 : 2;
+        wof[i] = wo;
+        to += (size_t)r * (M - r);
+        wo += (size_t)M * r;
+        if (orth) orth_boxes.push_back(i);
+      } else {
+        pt.mode = 1;
+      }
+      posts.push_back(pt);
+    }
+    if (!posts.empty()) {
+      DevBuf<PostTask> dt;
+      dt.upload(posts, s);
+      hipLaunchKernelGGL(k_post, dim3((unsigned)posts.size()), dim3(256), 0, s, dt.p);
+      MXG_HIP(hipGetLastError());
+      MXG_HIP(hipStreamSynchronize(s));
+    }
+    if (!orth_boxes.empty()) {
+      // P = triu(Q_in) W, QR(P), Q = full Q, R = triu (h2matrix.cpp:100-105)
+      size_t psz = 0, tz = 0;
+      for (long long i : orth_boxes) { psz += (size_t)Lv.dims[i] * Lv.ranks[i]; tz += Lv.ranks[i]; }
+      P.alloc(psz);
+      ptau.alloc(tz);
+      std::vector<GemmTask<double>> gt;
+      std::vector<GemmSeg<double>> gs;
+      std::vector<QrTask> pq;
+      std::vector<FormQTask> fq;
+      size_t po = 0, pt2 = 0;
+      for (long long i : orth_boxes) {
+        const int M = (int)Lv.dims[i], r = (int)Lv.ranks[i];
+        double* Pi = P.p + po;
+        gs.push_back(GemmSeg<double>{Lv.Q.p + Lv.offQ[i], Wm.p + wof[i], M, M, M, 0, 0, 1, 0});
+        gt.push_back(GemmTask<double>{Pi, M, M, r, (int)gs.size() - 1, 1, 1., 0.});
+        pq.push_back(QrTask{Pi, ptau.p + pt2, M, r, M, 0});
+        fq.push_back(FormQTask{Pi, ptau.p + pt2, Lv.Q.p + Lv.offQ[i], Lv.R.p + Lv.offQ[i], M, r, M, 0});
+        po += (size_t)M * r;
+        pt2 += r;
+      }
+      gemm_batched<double>(gt, gs, s);
+      qr_batched(pq, s);
+      formq_batched(fq, s);
+    }
+    b0 = b1;
+  }
+}
+
+void construct_coupling(mxg_h2_s* h, Level& Lv, int orth) {
+  hipStream_t s = h->ctx->stream;
+  const long long nb = Lv.nb;
+  if (Lv.C.n == 0) return;
+  DevBuf<double> K1, K2;
+  std::vector<GenTask> gens;
+  if (orth) {
+    K1.alloc(Lv.C.n);
+    K2.alloc(Lv.C.n);
+  }
+  for (long long y = 0; y < nb; y++)
+    for (long long ij = Lv.frp[y]; ij < Lv.frp[y + 1]; ij++) {
+      const long long x = Lv.fcol[ij];
+      const int ky = (int)Lv.ranks[y], kx = (int)Lv.ranks[x];
+      if (ky == 0 || kx == 0) continue;
+      double* out = (orth ? K1.p : Lv.C.p) + Lv.offC[ij];
+      gens.push_back(GenTask{Lv.S.p + Lv.offX[y], Lv.S.p + Lv.offX[x], 0, 0, out, ky, kx, ky, 0});
+    }
+  gen_tasks<double>(gens, h->bodies.p, h->k, s);
+  if (!orth) return;
+  std::vector<GemmTask<double>> t1, t2;
+  std::vector<GemmSeg<double>> s1, s2;
+  for (long long y = 0; y < nb; y++)
+    for (long long ij = Lv.frp[y]; ij < Lv.frp[y + 1]; ij++) {
+      const long long x = Lv.fcol[ij];
+      const int ky = (int)Lv.ranks[y], kx = (int)Lv.ranks[x];
+      if (ky == 0 || kx == 0) continue;
+      const double* Ry = Lv.R.p + Lv.offQ[y];
+      const double* Rx = Lv.R.p + Lv.offQ[x];
+      s1.push_back(GemmSeg<double>{Ry, K1.p + Lv.offC[ij], (int)Lv.dims[y], ky, ky, 0, 0, 1, 0});
+      t1.push_back(GemmTask<double>{K2.p + Lv.offC[ij], ky, ky, kx, (int)s1.size() - 1, 1, 1., 0.});
+      s2.push_back(GemmSeg<double>{K2.p + Lv.offC[ij], Rx, ky, (int)Lv.dims[x], kx, 0, 1, 0, 1});
+      t2.push_back(GemmTask<double>{Lv.C.p + Lv.offC[ij], ky, ky, kx, (int)s2.size() - 1, 1, 1., 0.});
+    }
+  gemm_batched<double>(t1, s1, s);
+  gemm_batched<double>(t2, s2, s);
+}
+
+void upload_indices(mxg_h2_s* h, Level& Lv) {
+  hipStream_t s = h->ctx->stream;
+  Lv.d_offX.upload(Lv.offX, s);
+  Lv.d_offZ.upload(Lv.offZ, s);
+  Lv.d_offQ.upload(Lv.offQ, s);
+  Lv.d_offA.upload(Lv.offA, s);
+  Lv.d_offC.upload(Lv.offC, s);
+  Lv.d_nrp.upload(Lv.nrp, s);
+  Lv.d_frp.upload(Lv.frp, s);
+  auto i32 = [](const std::vector<long long>& v) { std::vector<int> o(v.begin(), v.end()); return o; };
+  Lv.d_dims.upload(i32(Lv.dims), s);
+  Lv.d_ranks.upload(i32(Lv.ranks), s);
+  Lv.d_ncol.upload(i32(Lv.ncol), s);
+  Lv.d_fcol.upload(i32(Lv.fcol), s);
+  MXG_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+template <class T>
+void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y) {
+  hipStream_t s = h->ctx->stream;
+  const long long L = h->L;
+  auto Zp = [&](long long l) -> T* {
+    if constexpr (sizeof(T) == 8) return reinterpret_cast<T*>(h->Z[l].p); else return reinterpret_cast<T*>(h->Z32[l].p);
+  };
+  auto Wp = [&](long long l) -> T* {
+    if constexpr (sizeof(T) == 8) return reinterpret_cast<T*>(h->W[l].p); else return reinterpret_cast<T*>(h->W32[l].p);
+  };
+  auto Qp = [&](Level& Lv) -> const T* {
+    if constexpr (sizeof(T) == 8) return reinterpret_cast<const T*>(Lv.Q.p); else return reinterpret_cast<const T*>(Lv.Q32.p);
+  };
+  auto Cp = [&](Level& Lv) -> const T* {
+    if constexpr (sizeof(T) == 8) return reinterpret_cast<const T*>(Lv.C.p); else return reinterpret_cast<const T*>(Lv.C32.p);
+  };
+  auto Ap = [&](Level& Lv) -> const T* {
+    if constexpr (sizeof(T) == 8) return reinterpret_cast<const T*>(Lv.A.p); else return reinterpret_cast<const T*>(Lv.A32.p);
+  };
+  for (long long l = L; l >= 0; l--) {
+    Level& Lv = h->lv[l];
+    const T* xin = (l == L) ? x : Zp(l + 1);
+    if (Lv.lenZ == 0) continue;
+    hipLaunchKernelGGL(k_up<T>, dim3((unsigned)Lv.nb), dim3(256), 0, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p, Lv.d_offX.p,
+                       Lv.d_offZ.p, Qp(Lv), xin, Zp(l));
+    MXG_HIP(hipGetLastError());
+  }
+  for (long long l = 0; l <= L; l++) {
+    Level& Lv = h->lv[l];
+    const bool leaf = (l == L);
+    T* out = leaf ? y : Wp(l + 1);
+    size_t shm = std::max<long long>(Lv.maxK, 1) * sizeof(T);
+    if (leaf)
+      hipLaunchKernelGGL((k_down<T, true>), dim3((unsigned)Lv.nb), dim3(256), shm, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p,
+                         Lv.d_offX.p, Lv.d_offZ.p, Lv.d_frp.p, Lv.d_fcol.p, Lv.d_offC.p, Lv.d_nrp.p, Lv.d_ncol.p, Lv.d_offA.p,
+                         Qp(Lv), Cp(Lv), Ap(Lv), Zp(l), Wp(l), x, out);
+    else
+      hipLaunchKernelGGL((k_down<T, false>), dim3((unsigned)Lv.nb), dim3(256), shm, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p,
+                         Lv.d_offX.p, Lv.d_offZ.p, Lv.d_frp.p, Lv.d_fcol.p, Lv.d_offC.p, Lv.d_nrp.p, Lv.d_ncol.p, Lv.d_offA.p,
+                         Qp(Lv), Cp(Lv), Ap(Lv), Zp(l), Wp(l), x, out);
+    MXG_HIP(hipGetLastError());
+  }
+}
+
+template void h2_matvec_levels<double>(mxg_h2_s*, const double*, double*);
+template void h2_matvec_levels<float>(mxg_h2_s*, const float*, float*);
+
+}  // namespace mxg
+
+using namespace mxg;
+
+namespace {
+
+void make_shadow32(mxg_h2_s* h) {
+  if (h->shadow32) return;
+  hipStream_t s = h->ctx->stream;
+  h->Z32.resize(h->L + 1);
+  h->W32.resize(h->L + 1);
+  for (long long l = 0; l <= h->L; l++) {
+    Level& Lv = h->lv[l];
+    Lv.Q32.alloc(Lv.Q.n);
+    convert(Lv.Q32.p, Lv.Q.p, Lv.Q.n, s);
+    Lv.C32.alloc(Lv.C.n);
+    convert(Lv.C32.p, Lv.C.p, Lv.C.n, s);
+    if (l == h->L) {
+      Lv.A32.alloc(Lv.A.n);
+      convert(Lv.A32.p, Lv.A.p, Lv.A.n, s);
+    }
+    h->Z32[l].alloc(std::max<long long>(Lv.lenZ, 1));
+    h->W32[l].alloc_zero(std::max<long long>(Lv.lenZ, 1), s);
+  }
+  h->x32.alloc(h->n);
+  h->y32.alloc(h->n);
+  MXG_HIP(hipStreamSynchronize(s));
+  h->shadow32 = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mxg_h2_create(mxg_ctx_t ctx, mxg_kernel_t k, const double* bodies, long long nbodies, long long levels, int flags,
+                  mxg_h2_t* out) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx && out && bodies, "null argument");
+    MXG_REQUIRE(nbodies > 0 && nbodies < (1ll << 31), "nbodies out of range");
+    MXG_REQUIRE(levels >= 0 && levels < 31 && (1ll << levels) <= nbodies, "levels out of range");
+    MXG_REQUIRE(k.id >= MXG_KERNEL_LAPLACE && k.id <= MXG_KERNEL_GAUSSIAN, "unknown kernel id");
+    MXG_HIP(hipSetDevice(ctx->device));
+    auto* h = new mxg_h2_s();
+    try {
+      h->ctx = ctx;
+      h->k = k;
+      h->n = nbodies;
+      h->L = levels;
+      h->flags = flags;
+      h->bodies.upload(bodies, 3 * nbodies, ctx->stream);
+      h->lv.resize(levels + 1);
+      h->Z.resize(levels + 1);
+      h->W.resize(levels + 1);
+      MXG_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int mxg_h2_destroy(mxg_h2_t h) {
+  return guarded([&] {
+    if (!h) return;
+    (void)hipStreamSynchronize(h->ctx->stream);
+    delete h;
+  });
+}
+
+int mxg_h2_construct_level(mxg_h2_t h, long long l, long long nb, const long long* dims, const long long* leaf_begin,
+                           const long long* near_rp, const long long* near_col, const long long* far_rp,
+                           const long long* far_col, const long long* far_off, const long long* far_idx, double tol,
+                           long long maxrank, int orth, long long* ranks_out) {
+  return guarded([&] {
+    MXG_REQUIRE(h && dims && near_rp && far_rp && ranks_out, "null argument");
+    MXG_REQUIRE(l >= 0 && l <= h->L, "level out of range");
+    MXG_REQUIRE(nb == (1ll << l), "nboxes must be 2^level");
+    MXG_REQUIRE(l == h->L || h->lv[l + 1].built, "construct levels from the leaves upward");
+    MXG_REQUIRE(l < h->L || leaf_begin, "leaf level needs leaf_begin");
+    MXG_HIP(hipSetDevice(h->ctx->device));
+    hipStream_t s = h->ctx->stream;
+    Level& Lv = h->lv[l];
+    Level* low = (l < h->L) ? &h->lv[l + 1] : nullptr;
+    Lv = Level();
+    Lv.nb = nb;
+    Lv.dims.assign(dims, dims + nb);
+    Lv.nrp.assign(near_rp, near_rp + nb + 1);
+    Lv.ncol.assign(near_col, near_col + Lv.nrp[nb]);
+    Lv.frp.assign(far_rp, far_rp + nb + 1);
+    Lv.fcol.assign(far_col, far_col + Lv.frp[nb]);
+    MXG_REQUIRE(Lv.nrp[0] == 0 && Lv.frp[0] == 0, "CSR row pointers must start at 0");
+    for (long long t = 0; t < Lv.nrp[nb]; t++) MXG_REQUIRE(Lv.ncol[t] >= 0 && Lv.ncol[t] < nb, "near column out of range");
+    for (long long t = 0; t < Lv.frp[nb]; t++) MXG_REQUIRE(Lv.fcol[t] >= 0 && Lv.fcol[t] < nb, "far column out of range");
+    for (long long i = 0; i < nb; i++) {
+      MXG_REQUIRE(dims[i] >= 0 && dims[i] <= 4096, "box dimension out of range (max 4096)");
+      if (low) MXG_REQUIRE(dims[i] == low->ranks[2 * i] + low->ranks[2 * i + 1], "upper dims must be the sum of the child ranks");
+    }
+    if (!low) {
+      long long total = 0;
+      for (long long i = 0; i < nb; i++) {
+        MXG_REQUIRE(leaf_begin[i] >= 0 && leaf_begin[i] + dims[i] <= h->n, "leaf body range out of bounds");
+        total += dims[i];
+      }
+      MXG_REQUIRE(total == h->n, "leaf dims must sum to nbodies");
+    }
+    Lv.ranks.assign(nb, 0);
+    Lv.offX.assign(nb + 1, 0);
+    Lv.offQ.assign(nb + 1, 0);
+    Lv.diag.assign(nb, -1);
+    for (long long i = 0; i < nb; i++) {
+      Lv.offX[i + 1] = Lv.offX[i] + dims[i];
+      Lv.offQ[i + 1] = Lv.offQ[i] + dims[i] * dims[i];
+      Lv.maxM = std::max(Lv.maxM, dims[i]);
+      Lv.diag[i] = lookup(Lv.nrp, Lv.ncol, i, i);
+      MXG_REQUIRE(Lv.diag[i] >= 0, "every box needs its diagonal near block");
+    }
+    Lv.lenX = Lv.offX[nb];
+    const long long nnzA = Lv.nrp[nb];
+    Lv.offA.assign(nnzA + 1, 0);
+    for (long long i = 0; i < nb; i++)
+      for (long long t = Lv.nrp[i]; t < Lv.nrp[i + 1]; t++) Lv.offA[t + 1] = Lv.offA[t] + dims[i] * dims[Lv.ncol[t]];
+    Lv.up_off.assign(nnzA, -1);
+    Lv.up_ld.assign(nnzA, 0);
+
+    Lv.S.alloc(std::max<long long>(Lv.lenX, 1));
+    Lv.Q.alloc_zero(Lv.offQ[nb], s);
+    Lv.R.alloc_zero(Lv.offQ[nb], s);
+    const bool keepA = !low || (h->flags & MXG_H2_FACTORABLE);
+    if (keepA) Lv.A.alloc_zero(Lv.offA[nnzA], s);
+
+    if (!low) {
+      DevBuf<long long> d_begin;
+      d_begin.upload(leaf_begin, nb, s);
+      upload_indices(h, Lv);
+      hipLaunchKernelGGL(k_leaf_init, dim3((unsigned)nb), dim3(256), 0, s, (int)nb, Lv.d_offX.p, Lv.d_offQ.p, Lv.d_dims.p,
+                         d_begin.p, Lv.S.p, Lv.Q.p);
+      MXG_HIP(hipGetLastError());
+      std::vector<GenTask> gens;
+      for (long long i = 0; i < nb; i++)
+        for (long long t = Lv.nrp[i]; t < Lv.nrp[i + 1]; t++) {
+          const long long j = Lv.ncol[t];
+          gens.push_back(GenTask{nullptr, nullptr, leaf_begin[i], leaf_begin[j], Lv.A.p + Lv.offA[t], (int)dims[i], (int)dims[j],
+                                 (int)std::max<long long>(dims[i], 1), 0});
+        }
+      gen_tasks<double>(gens, h->bodies.p, h->k, s);
+    } else {
+      std::vector<const int*> ss;
+      std::vector<int*> sd;
+      std::vector<const double*> qs;
+      std::vector<double*> qd;
+      std::vector<int> sm, sn, sls, sld, qm, qn, qls, qld;
+      std::vector<const double*> cs;
+      std::vector<double*> cd;
+      std::vector<int> cm, cn, cls, cld;
+      for (long long i = 0; i < nb; i++) {
+        const long long M = dims[i];
+        for (int cy = 0; cy < 2; cy++) {
+          const long long y = 2 * i + cy;
+          const long long offy = cy ? low->ranks[2 * i] : 0, ny = low->ranks[y];
+          if (ny == 0) continue;
+          ss.push_back(low->S.p + low->offX[y]);
+          sd.push_back(Lv.S.p + Lv.offX[i] + offy);
+          sm.push_back((int)ny); sn.push_back(1); sls.push_back((int)ny); sld.push_back((int)ny);
+          qs.push_back(low->R.p + low->offQ[y]);
+          qd.push_back(Lv.Q.p + Lv.offQ[i] + offy + offy * M);
+          qm.push_back((int)ny); qn.push_back((int)ny); qls.push_back((int)low->dims[y]); qld.push_back((int)M);
+          for (long long ij = Lv.nrp[i]; ij < Lv.nrp[i + 1]; ij++) {
+            const long long j = Lv.ncol[ij];
+            for (int cx = 0; cx < 2; cx++) {
+              const long long x = 2 * j + cx;
+              const long long offx = cx ? low->ranks[2 * j] : 0, nx = low->ranks[x];
+              if (nx == 0) continue;
+              const long long lowN = lookup(low->nrp, low->ncol, y, x);
+              const long long lowC = lookup(low->frp, low->fcol, y, x);
+              const long long dp = Lv.offA[ij] + offy + offx * M;
+              if (lowN >= 0) {
+                low->up_off[lowN] = dp;
+                low->up_ld[lowN] = (int)M;
+              } else if (lowC >= 0 && keepA) {
+                cs.push_back(low->C.p + low->offC[lowC]);
+                cd.push_back(Lv.A.p + dp);
+                cm.push_back((int)ny); cn.push_back((int)nx); cls.push_back((int)ny); cld.push_back((int)M);
+              }
+            }
+          }
+        }
+      }
+      copy2d_batched<int>(ss, sd, sm, sn, sls, sld, s);
+      copy2d_batched<double>(qs, qd, qm, qn, qls, qld, s);
+      copy2d_batched<double>(cs, cd, cm, cn, cls, cld, s);
+    }
+
+    construct_basis(h, Lv, l, far_off, far_idx, tol, maxrank, orth);
+
+    Lv.offZ.assign(nb + 1, 0);
+    for (long long i = 0; i < nb; i++) {
+      Lv.offZ[i + 1] = Lv.offZ[i] + Lv.ranks[i];
+      Lv.maxK = std::max(Lv.maxK, Lv.ranks[i]);
+      ranks_out[i] = Lv.ranks[i];
+    }
+    Lv.lenZ = Lv.offZ[nb];
+    const long long nnzC = Lv.frp[nb];
+    Lv.offC.assign(nnzC + 1, 0);
+    for (long long i = 0; i < nb; i++)
+      for (long long t = Lv.frp[i]; t < Lv.frp[i + 1]; t++) Lv.offC[t + 1] = Lv.offC[t] + Lv.ranks[i] * Lv.ranks[Lv.fcol[t]];
+    Lv.C.alloc_zero(Lv.offC[nnzC], s);
+    construct_coupling(h, Lv, orth);
+    upload_indices(h, Lv);
+    h->Z[l].alloc_zero(std::max<long long>(Lv.lenZ, 1), s);
+    h->W[l].alloc_zero(std::max<long long>(Lv.lenZ, 1), s);
+    MXG_HIP(hipStreamSynchronize(s));
+    Lv.built = true;
+    h->shadow32 = false;
+  });
+}
+
+long long mxg_h2_level_size(mxg_h2_t h, long long l, int what) {
+  if (!h || l < 0 || l > h->L) return -1;
+  const Level& Lv = h->lv[l];
+  switch (what) {
+    case 0: return (long long)Lv.A.n;
+    case 1: return (long long)Lv.Q.n;
+    case 2: return (long long)Lv.R.n;
+    case 3: return (long long)Lv.C.n;
+    case 4: return Lv.lenX;
+    default: return -1;
+  }
+}
+
+int mxg_h2_download(mxg_h2_t h, long long l, int what, double* dst, long long count) {
+  return guarded([&] {
+    MXG_REQUIRE(h && dst, "null argument");
+    MXG_REQUIRE(l >= 0 && l <= h->L && h->lv[l].built, "level not constructed");
+    long long n = mxg_h2_level_size(h, l, what);
+    MXG_REQUIRE(n >= 0 && count == n, "count does not match the level size");
+    if (n == 0) return;
+    Level& Lv = h->lv[l];
+    hipStream_t s = h->ctx->stream;
+    if (what == 4) {
+      std::vector<int> tmp(n);
+      MXG_HIP(hipMemcpyAsync(tmp.data(), Lv.S.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+      MXG_HIP(hipStreamSynchronize(s));
+      for (long long i = 0; i < n; i++) dst[i] = tmp[i];
+      return;
+    }
+    const double* src = what == 0 ? Lv.A.p : what == 1 ? Lv.Q.p : what == 2 ? Lv.R.p : Lv.C.p;
+    MXG_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    MXG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int mxg_h2_matvec(mxg_h2_t h, const double* d_x, double* d_y, long long nrhs, long long ld, int precision) {
+  return guarded([&] {
+    MXG_REQUIRE(h && d_x && d_y, "null argument");
+    MXG_REQUIRE(nrhs >= 1 && ld >= h->n, "bad nrhs/ld");
+    for (long long l = 0; l <= h->L; l++) MXG_REQUIRE(h->lv[l].built, "H2 matrix not fully constructed");
+    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32, "unknown precision");
+    MXG_HIP(hipSetDevice(h->ctx->device));
+    hipStream_t s = h->ctx->stream;
+    if (precision == MXG_FP32) make_shadow32(h);
+    for (long long c = 0; c < nrhs; c++) {
+      const double* x = d_x + c * ld;
+      double* y = d_y + c * ld;
+      if (precision == MXG_FP64) {
+        h2_matvec_levels<double>(h, x, y);
+      } else {
+        convert(h->x32.p, x, h->n, s);
+        h2_matvec_levels<float>(h, h->x32.p, h->y32.p);
+        convert(y, h->y32.p, h->n, s);
+      }
+    }
+    MXG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

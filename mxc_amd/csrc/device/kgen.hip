@@ -1,0 +1,145 @@
+// Kernel-block generation and the matrix-free dense check.
+//
+// gen_matrix (kernel.cpp:71-85) evaluated on the device: one workgroup fills a
+// 64 x 32 tile of a column-major block; lane r owns a row (coalesced stores
+// down a column), the four waves take interleaved columns. Tasks of different
+// shapes share one launch through a prefix sum of their tile counts.
+#include "internal.hpp"
+
+namespace mxg {
+namespace {
+
+template <class T>
+__global__ void __launch_bounds__(256) k_gen(const GenTask* __restrict__ tasks, const long long* __restrict__ prefix,
+                                             int ntasks, const double* __restrict__ bodies, int kid, double diag,
+                                             double alpha) {
+  long long tile = blockIdx.x;
+  int t = find_task(prefix, ntasks, tile);
+  GenTask g = tasks[t];
+  long long local = tile - prefix[t];
+  int tiles_m = (g.m + GEN_TM - 1) / GEN_TM;
+  int r0 = (int)(local % tiles_m) * GEN_TM;
+  int c0 = (int)(local / tiles_m) * GEN_TN;
+  int r = r0 + (threadIdx.x & 63);
+  int cw = threadIdx.x >> 6;
+  if (r >= g.m) return;
+  long long br = g.ri ? g.ri[r] : g.rbeg + r;
+  double p[3] = {bodies[3 * br], bodies[3 * br + 1], bodies[3 * br + 2]};
+  T* out = reinterpret_cast<T*>(g.out);
+  int cend = min(c0 + GEN_TN, g.n);
+  for (int c = c0 + cw; c < cend; c += 4) {
+    long long bc = g.ci ? g.ci[c] : g.cbeg + c;
+    double q[3] = {bodies[3 * bc], bodies[3 * bc + 1], bodies[3 * bc + 2]};
+    out[r + (long long)c * g.ld] = (T)kernel_eval(kid, diag, alpha, dist2(p, q));
+  }
+}
+
+// Dense check: partial sums of y over column chunks, then a fixed-order
+// reduction (deterministic; no atomics).
+constexpr int DM_ROWS = 256, DM_CHUNK = 8192;
+
+__global__ void __launch_bounds__(256) k_dense_partial(const double* __restrict__ bodies, const double* __restrict__ x,
+                                                       long long n, int kid, double diag, double alpha,
+                                                       double* __restrict__ part) {
+  __shared__ double sp[256 * 3];
+  __shared__ double sx[256];
+  long long i = (long long)blockIdx.x * DM_ROWS + threadIdx.x;
+  long long j0 = (long long)blockIdx.y * DM_CHUNK;
+  long long j1 = min(j0 + DM_CHUNK, n);
+  double p[3] = {0., 0., 0.};
+  if (i < n) { p[0] = bodies[3 * i]; p[1] = bodies[3 * i + 1]; p[2] = bodies[3 * i + 2]; }
+  double acc = 0.;
+  for (long long jt = j0; jt < j1; jt += 256) {
+    int cnt = (int)min((long long)256, j1 - jt);
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) {
+      long long j = jt + threadIdx.x;
+      sp[3 * threadIdx.x] = bodies[3 * j];
+      sp[3 * threadIdx.x + 1] = bodies[3 * j + 1];
+      sp[3 * threadIdx.x + 2] = bodies[3 * j + 2];
+      sx[threadIdx.x] = x[j];
+    }
+    __syncthreads();
+    for (int c = 0; c < cnt; c++) acc += kernel_eval(kid, diag, alpha, dist2(p, &sp[3 * c])) * sx[c];
+  }
+  if (i < n) part[(long long)blockIdx.y * n + i] = acc;
+}
+
+__global__ void k_dense_reduce(const double* __restrict__ part, long long n, int nchunks, double* __restrict__ y) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.;
+  for (int c = 0; c < nchunks; c++) s += part[(long long)c * n + i];
+  y[i] = s;
+}
+
+}  // namespace
+
+template <class T>
+void gen_tasks(const std::vector<GenTask>& tasks, const double* d_bodies, mxg_kernel_t k, hipStream_t s) {
+  if (tasks.empty()) return;
+  std::vector<long long> prefix(tasks.size() + 1, 0);
+  for (size_t t = 0; t < tasks.size(); t++) {
+    long long tm = (tasks[t].m + GEN_TM - 1) / GEN_TM, tn = (tasks[t].n + GEN_TN - 1) / GEN_TN;
+    prefix[t + 1] = prefix[t] + ((tasks[t].m > 0 && tasks[t].n > 0) ? tm * tn : 0);
+  }
+  long long total = prefix.back();
+  if (total == 0) return;
+  MXG_REQUIRE(total < (1ll << 31), "generation grid too large");
+  DevBuf<GenTask> dt;
+  DevBuf<long long> dp;
+  dt.upload(tasks, s);
+  dp.upload(prefix, s);
+  hipLaunchKernelGGL(k_gen<T>, dim3((unsigned)total), dim3(256), 0, s, dt.p, dp.p, (int)tasks.size(), d_bodies, k.id, k.diag,
+                     k.alpha);
+  MXG_HIP(hipGetLastError());
+  MXG_HIP(hipStreamSynchronize(s));  // task arrays are freed on return
+}
+
+template void gen_tasks<double>(const std::vector<GenTask>&, const double*, mxg_kernel_t, hipStream_t);
+template void gen_tasks<float>(const std::vector<GenTask>&, const double*, mxg_kernel_t, hipStream_t);
+
+}  // namespace mxg
+
+using namespace mxg;
+
+extern "C" {
+
+int mxg_gen_block(mxg_ctx_t ctx, mxg_kernel_t k, long long m, long long n, const double* d_bi, const double* d_bj,
+                  double* d_A, long long lda) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    MXG_REQUIRE(m >= 0 && n >= 0 && lda >= (m > 0 ? m : 1), "bad block shape");
+    MXG_REQUIRE(m < (1ll << 31) && n < (1ll << 31) && lda < (1ll << 31), "block too large");
+    if (m == 0 || n == 0) return;
+    MXG_REQUIRE(d_bi && d_bj && d_A, "null pointer");
+    // Points of the two sides are separate arrays: stage them as one body array.
+    DevBuf<double> pts;
+    pts.alloc(3 * (size_t)(m + n));
+    MXG_HIP(hipMemcpyAsync(pts.p, d_bi, 3 * m * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    MXG_HIP(hipMemcpyAsync(pts.p + 3 * m, d_bj, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    GenTask g{nullptr, nullptr, 0, m, d_A, (int)m, (int)n, (int)lda, 0};
+    gen_tasks<double>({g}, pts.p, k, ctx->stream);
+  });
+}
+
+int mxg_dense_matvec(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n, const double* d_x, double* d_y) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    MXG_REQUIRE(n >= 0, "negative size");
+    if (n == 0) return;
+    MXG_REQUIRE(d_bodies && d_x && d_y, "null pointer");
+    int nchunks = (int)ceil_div(n, DM_CHUNK);
+    MXG_REQUIRE(ceil_div(n, DM_ROWS) < (1u << 31) && nchunks < 65536, "dense check size too large");
+    DevBuf<double> part;
+    part.alloc((size_t)nchunks * n);
+    hipLaunchKernelGGL(k_dense_partial, dim3(ceil_div(n, DM_ROWS), nchunks), dim3(256), 0, ctx->stream, d_bodies, d_x, n, k.id,
+                       k.diag, k.alpha, part.p);
+    MXG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_dense_reduce, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, part.p, n, nchunks, d_y);
+    MXG_HIP(hipGetLastError());
+    MXG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,350 @@
+// Dense factorizations, one 256-thread workgroup per matrix, variable sizes.
+//
+//  * qr:   Householder QR (Eigen HouseholderQR reflector convention,
+//          h2matrix.cpp:78-79,102-105): beta = -sign(x0)||x||,
+//          v = [1; x_tail/(x0 - beta)], tau = (beta - x0)/beta.
+//  * cpqr: column-pivoted QR with Eigen ColPivHouseholderQR semantics
+//          (h2matrix.cpp:84-89): first-index argmax of the updated column
+//          norms, LAPACK xGEQPF norm downdating with recomputation below
+//          sqrt(eps), the "nonzero pivots" cut and rank() = #{|R_ii| > tol *
+//          max pivot}. Stops as soon as the rank is settled.
+//  * lu_inverse: LU with partial pivoting (Eigen PartialPivLU,
+//          h2matrix.cpp:435-441) followed by the explicit inverse from the
+//          triangular factors; flags |det| <= DBL_MIN through log|det|.
+// Each column operation maps a wave to a column with lanes down the rows, so
+// global accesses are coalesced; workgroup reductions go through LDS.
+#include "internal.hpp"
+
+#include <cfloat>
+
+namespace mxg {
+namespace {
+
+struct Red {
+  double v[4];
+  int i[4];
+};
+
+__device__ inline double wg_sum(double v, Red& r) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) r.v[wave] = v;
+  __syncthreads();
+  return (r.v[0] + r.v[1]) + (r.v[2] + r.v[3]);
+}
+
+// max of v over the workgroup; ties resolved to the smallest index.
+__device__ inline void wg_argmax(double v, int idx, Red& r, double& bv, int& bi) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double ov = __shfl_xor(v, o, 64);
+    int oi = __shfl_xor(idx, o, 64);
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) { r.v[wave] = v; r.i[wave] = idx; }
+  __syncthreads();
+  bv = r.v[0];
+  bi = r.i[0];
+  for (int w = 1; w < 4; w++)
+    if (r.v[w] > bv || (r.v[w] == bv && r.i[w] < bi)) { bv = r.v[w]; bi = r.i[w]; }
+}
+
+// Householder reflector of x[0..len) (stride 1) in place; returns beta, writes tau.
+// All threads participate; x[0] is left for the caller to overwrite with beta.
+__device__ inline double make_reflector(double* x, int len, Red& r, double& tau) {
+  double s = 0.;
+  for (int i = 1 + threadIdx.x; i < len; i += 256) s += x[i] * x[i];
+  s = wg_sum(s, r);
+  const double c0 = x[0];
+  double beta;
+  if (s <= DBL_MIN) {
+    tau = 0.;
+    beta = c0;
+    for (int i = 1 + threadIdx.x; i < len; i += 256) x[i] = 0.;
+  } else {
+    beta = sqrt(c0 * c0 + s);
+    if (c0 >= 0.) beta = -beta;
+    const double sc = 1. / (c0 - beta);
+    for (int i = 1 + threadIdx.x; i < len; i += 256) x[i] *= sc;
+    tau = (beta - c0) / beta;
+  }
+  return beta;
+}
+
+// Apply H = I - tau [1; v][1; v]^T (v = x[1..len)) to columns c of the block
+// starting at row 0 of `base` (ld) for c in [c0, c1). Wave per column.
+__device__ inline void apply_reflector(const double* x, int len, double tau, double* base, long long ld, int c0, int c1) {
+  if (tau == 0.) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int c = c0 + wave; c < c1; c += 4) {
+    double* col = base + (long long)c * ld;
+    double w = (lane == 0) ? col[0] : 0.;
+    for (int i = 1 + lane; i < len; i += 64) w += x[i] * col[i];
+    w = wave_sum(w) * tau;
+    if (lane == 0) col[0] -= w;
+    for (int i = 1 + lane; i < len; i += 64) col[i] -= w * x[i];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_qr(const QrTask* __restrict__ tasks) {
+  __shared__ Red red;
+  __shared__ double sh_tau;
+  const QrTask t = tasks[blockIdx.x];
+  const int size = min(t.m, t.n);
+  for (int k = 0; k < size; k++) {
+    double* col = t.A + k + (long long)k * t.lda;
+    const int len = t.m - k;
+    double tau;
+    const double beta = make_reflector(col, len, red, tau);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      col[0] = beta;
+      t.tau[k] = tau;
+      sh_tau = tau;
+    }
+    __syncthreads();
+    apply_reflector(col, len, sh_tau, t.A + k, t.lda, k + 1, t.n);
+    __syncthreads();
+  }
+  if (t.zero_lower)
+    for (long long e = threadIdx.x; e < (long long)size * size; e += 256) {
+      const int i = (int)(e % size), j = (int)(e / size);
+      if (i > j) t.A[i + (long long)j * t.lda] = 0.;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cpqr(const CpqrTask* __restrict__ tasks) {
+  __shared__ Red red;
+  __shared__ double sh_tau, sh_maxpivot;
+  const CpqrTask t = tasks[blockIdx.x];
+  const int K = t.K, M = t.M, size = min(K, M);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double* upd = t.work;
+  double* dir = t.work + M;
+  for (int j = threadIdx.x; j < M; j += 256) t.perm[j] = j;
+  for (int j = wave; j < M; j += 4) {
+    const double* col = t.A + (long long)j * t.lda;
+    double s = 0.;
+    for (int i = lane; i < K; i += 64) s += col[i] * col[i];
+    s = wave_sum(s);
+    if (lane == 0) upd[j] = dir[j] = sqrt(s);
+  }
+  __syncthreads();
+  double mx = 0.;
+  for (int j = threadIdx.x; j < M; j += 256) mx = fmax(mx, upd[j]);
+  double maxnorm;
+  int dummy;
+  wg_argmax(mx, 0, red, maxnorm, dummy);
+  const double eps = DBL_EPSILON;
+  const double th_helper = (maxnorm * eps) * (maxnorm * eps) / (double)K;
+  const double downdate_th = sqrt(eps);
+  int nonzero = size, done = 0;
+  double maxpivot = 0.;
+  for (int k = 0; k < t.steps; k++) {
+    double bv = -1.;
+    int bi = M;
+    for (int j = k + threadIdx.x; j < M; j += 256)
+      if (upd[j] > bv) { bv = upd[j]; bi = j; }
+    double gv;
+    int gi;
+    wg_argmax(bv, bi, red, gv, gi);
+    if (nonzero == size && gv * gv < th_helper * (double)(K - k)) nonzero = k;
+    if (t.tol > 0. && nonzero < size) break;                      // rank counts i < nonzero only
+    if (t.tol > 0. && k > 0 && gv <= t.tol * maxpivot) break;    // every later |R_ii| is below the threshold
+    if (gi != k) {
+      double* ck = t.A + (long long)k * t.lda;
+      double* cb = t.A + (long long)gi * t.lda;
+      for (int i = threadIdx.x; i < K; i += 256) {
+        double a = ck[i];
+        ck[i] = cb[i];
+        cb[i] = a;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double a = upd[k]; upd[k] = upd[gi]; upd[gi] = a;
+        a = dir[k]; dir[k] = dir[gi]; dir[gi] = a;
+        int p = t.perm[k]; t.perm[k] = t.perm[gi]; t.perm[gi] = p;
+      }
+    }
+    __syncthreads();
+    double* col = t.A + k + (long long)k * t.lda;
+    const int len = K - k;
+    double tau;
+    const double beta = make_reflector(col, len, red, tau);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      col[0] = beta;
+      sh_tau = tau;
+      sh_maxpivot = fmax(maxpivot, fabs(beta));
+    }
+    __syncthreads();
+    maxpivot = sh_maxpivot;
+    apply_reflector(col, len, sh_tau, t.A + k, t.lda, k + 1, M);
+    __syncthreads();
+    for (int j = k + 1 + threadIdx.x; j < M; j += 256) {
+      if (upd[j] != 0.) {
+        double q = fabs(t.A[k + (long long)j * t.lda]) / upd[j];
+        q = (1. + q) * (1. - q);
+        q = q < 0. ? 0. : q;
+        const double r = upd[j] / dir[j];
+        if (q * r * r <= downdate_th) {
+          const double* cj = t.A + (long long)j * t.lda;
+          double s = 0.;
+          for (int i = k + 1; i < K; i++) s += cj[i] * cj[i];
+          dir[j] = upd[j] = sqrt(s);
+        } else {
+          upd[j] *= sqrt(q);
+        }
+      }
+    }
+    __syncthreads();
+    done = k + 1;
+  }
+  if (threadIdx.x == 0) {
+    int rank;
+    if (t.tol > 0.) {
+      const double th = maxpivot * t.tol;
+      const int lim = min(nonzero, done);
+      rank = 0;
+      for (int i = 0; i < lim; i++) rank += fabs(t.A[i + (long long)i * t.lda]) > th;
+    } else {
+      rank = done;
+    }
+    if (t.maxrank > 0) rank = min(rank, t.maxrank);
+    *t.rank_out = rank;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_formq(const FormQTask* __restrict__ tasks) {
+  const FormQTask t = tasks[blockIdx.x];
+  const int M = t.M;
+  for (long long e = threadIdx.x; e < (long long)M * M; e += 256) {
+    const int i = (int)(e % M), j = (int)(e / M);
+    t.Q[e] = (i == j) ? 1. : 0.;
+    t.R[e] = (j < t.r && i <= j) ? t.P[i + (long long)j * t.ldp] : 0.;
+  }
+  __syncthreads();
+  for (int k = t.r - 1; k >= 0; k--) {
+    apply_reflector(t.P + k + (long long)k * t.ldp, M - k, t.tau[k], t.Q + k, M, k, M);
+    __syncthreads();
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_lu_inverse(const LuTask<T>* __restrict__ tasks) {
+  __shared__ Red red;
+  __shared__ int ipiv[1024];
+  const LuTask<T> t = tasks[blockIdx.x];
+  const int n = t.n;
+  T* W = t.work;
+  T* X = t.inv;
+  for (long long e = threadIdx.x; e < (long long)n * n; e += 256) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    W[e] = t.A[i + (long long)j * t.lda];
+  }
+  __syncthreads();
+  double logdet = 0.;
+  int singular = 0;
+  for (int k = 0; k < n; k++) {
+    double bv = -1.;
+    int bi = n;
+    for (int i = k + threadIdx.x; i < n; i += 256) {
+      const double a = fabs((double)W[i + (long long)k * n]);
+      if (a > bv) { bv = a; bi = i; }
+    }
+    double gv;
+    int gi;
+    wg_argmax(bv, bi, red, gv, gi);
+    if (threadIdx.x == 0) ipiv[k] = gi;
+    if (gi != k)
+      for (int c = threadIdx.x; c < n; c += 256) {
+        T a = W[k + (long long)c * n];
+        W[k + (long long)c * n] = W[gi + (long long)c * n];
+        W[gi + (long long)c * n] = a;
+      }
+    __syncthreads();
+    const T d = W[k + (long long)k * n];
+    if (d == T(0)) {
+      singular = 1;
+      continue;
+    }
+    logdet += log(fabs((double)d));
+    const T inv = T(1) / d;
+    for (int i = k + 1 + threadIdx.x; i < n; i += 256) W[i + (long long)k * n] *= inv;
+    __syncthreads();
+    const int r = n - k - 1;
+    for (long long e = threadIdx.x; e < (long long)r * r; e += 256) {
+      const int i = k + 1 + (int)(e % r), j = k + 1 + (int)(e / r);
+      W[i + (long long)j * n] -= W[i + (long long)k * n] * W[k + (long long)j * n];
+    }
+    __syncthreads();
+  }
+  // X = P I, then L^{-1}, then U^{-1}
+  for (long long e = threadIdx.x; e < (long long)n * n; e += 256) X[e] = ((e % n) == (e / n)) ? T(1) : T(0);
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    const int p = ipiv[k];
+    if (p != k)
+      for (int c = threadIdx.x; c < n; c += 256) {
+        T a = X[k + (long long)c * n];
+        X[k + (long long)c * n] = X[p + (long long)c * n];
+        X[p + (long long)c * n] = a;
+      }
+    __syncthreads();
+  }
+  for (int k = 0; k < n - 1; k++) {
+    const int r = n - k - 1;
+    for (long long e = threadIdx.x; e < (long long)r * n; e += 256) {
+      const int i = k + 1 + (int)(e % r), c = (int)(e / r);
+      X[i + (long long)c * n] -= W[i + (long long)k * n] * X[k + (long long)c * n];
+    }
+    __syncthreads();
+  }
+  for (int k = n - 1; k >= 0; k--) {
+    const T d = W[k + (long long)k * n];
+    for (int c = threadIdx.x; c < n; c += 256) X[k + (long long)c * n] /= d;
+    __syncthreads();
+    for (long long e = threadIdx.x; e < (long long)k * n; e += 256) {
+      const int i = (int)(e % k), c = (int)(e / k);
+      X[i + (long long)c * n] -= W[i + (long long)k * n] * X[k + (long long)c * n];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *t.info = (singular || logdet <= log(DBL_MIN)) ? 1 : 0;
+}
+
+template <class Task, class Kern>
+void launch_tasks(const std::vector<Task>& tasks, Kern kern, hipStream_t s) {
+  if (tasks.empty()) return;
+  MXG_REQUIRE(tasks.size() < (1u << 31), "too many tasks");
+  DevBuf<Task> dt;
+  dt.upload(tasks, s);
+  hipLaunchKernelGGL(kern, dim3((unsigned)tasks.size()), dim3(256), 0, s, dt.p);
+  MXG_HIP(hipGetLastError());
+  MXG_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+void qr_batched(const std::vector<QrTask>& tasks, hipStream_t s) { launch_tasks(tasks, k_qr, s); }
+
+void formq_batched(const std::vector<FormQTask>& tasks, hipStream_t s) { launch_tasks(tasks, k_formq, s); }
+
+void cpqr_batched(const std::vector<CpqrTask>& tasks, hipStream_t s) {
+  for (const CpqrTask& t : tasks) MXG_REQUIRE(t.steps <= std::min(t.K, t.M), "cpqr steps exceed min(K, M)");
+  launch_tasks(tasks, k_cpqr, s);
+}
+
+template <class T>
+void lu_inverse_batched(const std::vector<LuTask<T>>& tasks, hipStream_t s) {
+  for (const LuTask<T>& t : tasks) MXG_REQUIRE(t.n >= 0 && t.n <= 1024, "LU block larger than 1024");
+  launch_tasks(tasks, k_lu_inverse<T>, s);
+}
+
+template void lu_inverse_batched<double>(const std::vector<LuTask<double>>&, hipStream_t);
+template void lu_inverse_batched<float>(const std::vector<LuTask<float>>&, hipStream_t);
+
+}  // namespace mxg
