@@ -144,7 +144,7 @@ def h2() -> C.CDLL:
         L.mxc_solver_device.restype = P
         L.mxc_solver_ulv.argtypes = [P]
         L.mxc_solver_ulv.restype = P
-        L.mxc_csr.argtypes = [c_int, c_d, c_ll, P, P, P, P, P, c_ll]
+        L.mxc_csr.argtypes = [c_int, c_d, c_ll, P, P, P, P, P, P, c_ll]
         L.mxc_csr.restype = c_ll
         L.mxc_far_sample.argtypes = [c_ll, P, P, P, c_ll, c_ll, P, c_ll]
         L.mxc_far_sample.restype = c_ll

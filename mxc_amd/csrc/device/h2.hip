@@ -442,11 +442,13 @@ void upload_indices(mxg_h2_s* h, Level& Lv) {
   Lv.d_offC.upload(Lv.offC, s);
   Lv.d_nrp.upload(Lv.nrp, s);
   Lv.d_frp.upload(Lv.frp, s);
-  auto i32 = [](const std::vector<long long>& v) { std::vector<int> o(v.begin(), v.end()); return o; };
-  Lv.d_dims.upload(i32(Lv.dims), s);
-  Lv.d_ranks.upload(i32(Lv.ranks), s);
-  Lv.d_ncol.upload(i32(Lv.ncol), s);
-  Lv.d_fcol.upload(i32(Lv.fcol), s);
+  // host staging must outlive the asynchronous copies: synchronize before it goes
+  std::vector<int> dims(Lv.dims.begin(), Lv.dims.end()), ranks(Lv.ranks.begin(), Lv.ranks.end());
+  std::vector<int> ncol(Lv.ncol.begin(), Lv.ncol.end()), fcol(Lv.fcol.begin(), Lv.fcol.end());
+  Lv.d_dims.upload(dims, s);
+  Lv.d_ranks.upload(ranks, s);
+  Lv.d_ncol.upload(ncol, s);
+  Lv.d_fcol.upload(fcol, s);
   MXG_HIP(hipStreamSynchronize(s));
 }
 

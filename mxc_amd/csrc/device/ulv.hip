@@ -1,0 +1,500 @@
+// ULV factorization and substitution of an H^2 matrix on the device, in FP64
+// or FP32 (templated on the factor precision T).
+//
+// Factorization of one level (H2Matrix::factorize, h2matrix.cpp:346-425, and
+// its batched formulation device_factorize.cu:56-142), for boxes i with
+// M = dims, s = rank, r = M - s, and near pairs (i, j):
+//   1  A_ii <- U_i^T A_ii U_i                          (two batched GEMMs)
+//   2  rr_i^{-1} = (A_ii[s:, s:])^{-1}                 (batched LU + inverse)
+//   3  V_i = [U_i[:, :s]^T ; rr_i^{-1} U_i[:, s:]^T]   (stored for the solves)
+//   4  A_ii[s:, :s] <- rr^{-1} A_ii[s:, :s];  A_ii[:s, :s] -= A_ii[:s, s:] A_ii[s:, :s]
+//   5  A_ij <- V_i A_ij U_j for j != i, only the ss, sr and rs blocks (the rr
+//      block of an off-diagonal pair is never read again)
+//   6  A_ij[:s, :s] -= A_ii.sr A_ij.rs + A_ij.sr A_jj.rs   (j != i)
+//      A_ii[:s, :s] -= sum_j (A_ij.sr rr_j^{-1}) A_ij.sr^T  (single Schur
+//      complement onto the skeleton, no fill-in between non-neighbours)
+//   7  A_ij[:s, :s] -> the parent's near block (factorizeCopyNext,
+//      h2matrix.cpp:427-446)
+// Every step is one batched launch over the whole level.
+//
+// Substitution (h2matrix.cpp:448-537): forward L -> 0, x_i = V_i y_i,
+// z_i = x_i[:s] - sum_j A_ij.sr x_j[s_j:]; backward 0 -> L,
+// x_i = [w_i ; x_i[s:] - sum_j A_ij.rs w_j], y_i = U_i x_i. Vectors hand over
+// between levels in place (a level's z is its parent's y, the parent's y is
+// the child level's w), as in the matvec.
+#include <chrono>
+#include <memory>
+
+#include "h2.hpp"
+
+namespace mxg {
+namespace {
+
+template <class T>
+__global__ void k_transpose_rows(const int* __restrict__ dims, const int* __restrict__ ranks, const long long* __restrict__ offQ,
+                                 const T* __restrict__ U, T* __restrict__ V) {
+  // V[:s, :] = U[:, :s]^T
+  const int i = blockIdx.x;
+  const int M = dims[i], s = ranks[i];
+  const T* Ui = U + offQ[i];
+  T* Vi = V + offQ[i];
+  for (long long e = threadIdx.x; e < (long long)s * M; e += blockDim.x) {
+    const int r = (int)(e % s), c = (int)(e / s);
+    Vi[r + (long long)c * M] = Ui[c + (long long)r * M];
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_fwd1(const int* __restrict__ dims, const long long* __restrict__ offQ,
+                                              const long long* __restrict__ offX, const T* __restrict__ V,
+                                              const T* __restrict__ y, T* __restrict__ x) {
+  const int i = blockIdx.x;
+  const int M = dims[i];
+  const T* Vi = V + offQ[i];
+  const T* yi = y + offX[i];
+  for (int r = threadIdx.x; r < M; r += 256) {
+    T s = 0;
+    for (int c = 0; c < M; c++) s += Vi[r + (long long)c * M] * yi[c];
+    x[offX[i] + r] = s;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_fwd2(const int* __restrict__ dims, const int* __restrict__ ranks,
+                                              const long long* __restrict__ offX, const long long* __restrict__ offZ,
+                                              const long long* __restrict__ nrp, const int* __restrict__ ncol,
+                                              const long long* __restrict__ offA, const T* __restrict__ A,
+                                              const T* __restrict__ x, T* __restrict__ z) {
+  const int i = blockIdx.x;
+  const int M = dims[i], s = ranks[i];
+  for (int r = threadIdx.x; r < s; r += 256) {
+    T acc = x[offX[i] + r];
+    for (long long ij = nrp[i]; ij < nrp[i + 1]; ij++) {
+      const int j = ncol[ij];
+      const int N = dims[j], sj = ranks[j];
+      const T* a = A + offA[ij] + r + (long long)sj * M;
+      const T* xj = x + offX[j] + sj;
+      for (int c = 0; c < N - sj; c++) acc -= a[(long long)c * M] * xj[c];
+    }
+    z[offZ[i] + r] = acc;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_bwd(const int* __restrict__ dims, const int* __restrict__ ranks,
+                                             const long long* __restrict__ offQ, const long long* __restrict__ offX,
+                                             const long long* __restrict__ offZ, const long long* __restrict__ nrp,
+                                             const int* __restrict__ ncol, const long long* __restrict__ offA,
+                                             const T* __restrict__ A, const T* __restrict__ U, const T* __restrict__ x,
+                                             const T* __restrict__ w, T* __restrict__ y) {
+  extern __shared__ unsigned char smem[];
+  T* sx = reinterpret_cast<T*>(smem);
+  const int i = blockIdx.x;
+  const int M = dims[i], s = ranks[i];
+  for (int r = threadIdx.x; r < M; r += 256) {
+    T acc;
+    if (r < s) {
+      acc = w[offZ[i] + r];
+    } else {
+      acc = x[offX[i] + r];
+      for (long long ij = nrp[i]; ij < nrp[i + 1]; ij++) {
+        const int j = ncol[ij];
+        const int sj = ranks[j];
+        const T* a = A + offA[ij] + r;
+        const T* wj = w + offZ[j];
+        for (int c = 0; c < sj; c++) acc -= a[(long long)c * M] * wj[c];
+      }
+    }
+    sx[r] = acc;
+  }
+  __syncthreads();
+  const T* Ui = U + offQ[i];
+  for (int r = threadIdx.x; r < M; r += 256) {
+    T acc = 0;
+    for (int c = 0; c < M; c++) acc += Ui[r + (long long)c * M] * sx[c];
+    y[offX[i] + r] = acc;
+  }
+}
+
+// Scratch budget (elements) of one off-diagonal chunk of step 5.
+constexpr size_t PAIR_BUDGET = size_t(1) << 28;
+
+}  // namespace
+
+struct UlvBase {
+  virtual ~UlvBase() = default;
+  virtual void factorize() = 0;
+  virtual void solve(double* x) = 0;
+  virtual void info(int* out) = 0;
+};
+
+template <class T>
+struct UlvImpl : UlvBase {
+  mxg_h2_s* h;
+  struct Lv {
+    DevBuf<T> A, U, V, RRinv, X, Z;
+    std::vector<long long> offRR;
+    DevBuf<int> info;
+    int info_sum = 0;
+  };
+  std::vector<Lv> lv;
+  DevBuf<T> vin;
+
+  explicit UlvImpl(mxg_h2_s* hh) : h(hh) {
+    hipStream_t s = h->ctx->stream;
+    lv.resize(h->L + 1);
+    for (long long l = 0; l <= h->L; l++) {
+      Level& H = h->lv[l];
+      Lv& F = lv[l];
+      MXG_REQUIRE(H.A.n == (size_t)H.offA[H.nrp.back()], "H2 matrix was built without MXG_H2_FACTORABLE");
+      F.A.alloc(H.A.n);
+      convert(F.A.p, H.A.p, H.A.n, s);
+      F.U.alloc(H.Q.n);
+      convert(F.U.p, H.Q.p, H.Q.n, s);
+      F.V.alloc_zero(H.Q.n, s);
+      F.offRR.assign(H.nb + 1, 0);
+      for (long long i = 0; i < H.nb; i++) {
+        const long long r = H.dims[i] - H.ranks[i];
+        F.offRR[i + 1] = F.offRR[i] + r * r;
+      }
+      F.RRinv.alloc(std::max<long long>(F.offRR.back(), 1));
+      F.X.alloc(std::max<long long>(H.lenX, 1));
+      F.Z.alloc(std::max<long long>(H.lenZ, 1));
+      F.info.alloc_zero(std::max<long long>(H.nb, 1), s);
+    }
+    vin.alloc(h->n);
+    MXG_HIP(hipStreamSynchronize(s));
+  }
+
+  void factorize() override {
+    for (long long l = h->L; l >= 0; l--) {
+      factor_level(l);
+      if (l > 0) copy_next(l);
+    }
+    hipStream_t s = h->ctx->stream;
+    for (long long l = 0; l <= h->L; l++) {
+      std::vector<int> inf(h->lv[l].nb);
+      MXG_HIP(hipMemcpyAsync(inf.data(), lv[l].info.p, inf.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+      MXG_HIP(hipStreamSynchronize(s));
+      lv[l].info_sum = 0;
+      for (int v : inf) lv[l].info_sum += v;
+    }
+  }
+
+  void factor_level(long long l) {
+    hipStream_t s = h->ctx->stream;
+    Level& H = h->lv[l];
+    Lv& F = lv[l];
+    const long long nb = H.nb;
+    auto M_ = [&](long long i) { return (int)H.dims[i]; };
+    auto S_ = [&](long long i) { return (int)H.ranks[i]; };
+    T* A = F.A.p;
+    T* U = F.U.p;
+    T* V = F.V.p;
+    auto Aij = [&](long long ij) { return A + H.offA[ij]; };
+    auto Ui = [&](long long i) { return U + H.offQ[i]; };
+    auto Vi = [&](long long i) { return V + H.offQ[i]; };
+    auto RRi = [&](long long i) { return F.RRinv.p + F.offRR[i]; };
+
+    // 1: two-sided rotation of the diagonal blocks
+    {
+      DevBuf<T> T1;
+      T1.alloc(std::max<long long>(H.offQ[nb], 1));
+      std::vector<GemmTask<T>> t1, t2;
+      std::vector<GemmSeg<T>> s1, s2;
+      for (long long i = 0; i < nb; i++) {
+        const int M = M_(i);
+        if (M == 0) continue;
+        s1.push_back(GemmSeg<T>{Aij(H.diag[i]), Ui(i), M, M, M, 0, 0, 0, 0});
+        t1.push_back(GemmTask<T>{T1.p + H.offQ[i], M, M, M, (int)s1.size() - 1, 1, 1., 0.});
+        s2.push_back(GemmSeg<T>{Ui(i), T1.p + H.offQ[i], M, M, M, 1, 0, 0, 0});
+        t2.push_back(GemmTask<T>{Aij(H.diag[i]), M, M, M, (int)s2.size() - 1, 1, 1., 0.});
+      }
+      gemm_batched<T>(t1, s1, s);
+      gemm_batched<T>(t2, s2, s);
+    }
+    // 2: rr inverses
+    {
+      size_t wsz = 0;
+      for (long long i = 0; i < nb; i++) {
+        const long long r = M_(i) - S_(i);
+        wsz += (size_t)r * r + 2 * r;
+      }
+      DevBuf<T> work;
+      work.alloc(std::max<size_t>(wsz, 1));
+      std::vector<LuTask<T>> lu;
+      size_t wo = 0;
+      for (long long i = 0; i < nb; i++) {
+        const int M = M_(i), sI = S_(i), r = M - sI;
+        if (r == 0) continue;
+        lu.push_back(LuTask<T>{Aij(H.diag[i]) + sI + (long long)sI * M, RRi(i), work.p + wo, F.info.p + i, r, M});
+        wo += (size_t)r * r + 2 * r;
+      }
+      lu_inverse_batched<T>(lu, s);
+    }
+    // 3: V
+    {
+      hipLaunchKernelGGL(k_transpose_rows<T>, dim3((unsigned)nb), dim3(256), 0, s, H.d_dims.p, H.d_ranks.p, H.d_offQ.p, U, V);
+      MXG_HIP(hipGetLastError());
+      std::vector<GemmTask<T>> t;
+      std::vector<GemmSeg<T>> g;
+      for (long long i = 0; i < nb; i++) {
+        const int M = M_(i), sI = S_(i), r = M - sI;
+        if (r == 0) continue;
+        g.push_back(GemmSeg<T>{RRi(i), Ui(i) + (long long)sI * M, r, M, r, 0, 1, 0, 0});
+        t.push_back(GemmTask<T>{Vi(i) + sI, M, r, M, (int)g.size() - 1, 1, 1., 0.});
+      }
+      gemm_batched<T>(t, g, s);
+    }
+    // 4: rs <- rr^{-1} rs; ss -= sr rs
+    {
+      size_t tsz = 0;
+      for (long long i = 0; i < nb; i++) tsz += (size_t)(M_(i) - S_(i)) * S_(i);
+      DevBuf<T> T2;
+      T2.alloc(std::max<size_t>(tsz, 1));
+      std::vector<GemmTask<T>> t1, t2;
+      std::vector<GemmSeg<T>> s1, s2;
+      std::vector<const T*> cs;
+      std::vector<T*> cd;
+      std::vector<int> cm, cn, cls, cld;
+      size_t to = 0;
+      for (long long i = 0; i < nb; i++) {
+        const int M = M_(i), sI = S_(i), r = M - sI;
+        if (r == 0 || sI == 0) continue;
+        T* Aii = Aij(H.diag[i]);
+        s1.push_back(GemmSeg<T>{RRi(i), Aii + sI, r, M, r, 0, 0, 0, 0});
+        t1.push_back(GemmTask<T>{T2.p + to, r, r, sI, (int)s1.size() - 1, 1, 1., 0.});
+        s2.push_back(GemmSeg<T>{Aii + (long long)sI * M, T2.p + to, M, r, r, 0, 0, 0, 0});
+        t2.push_back(GemmTask<T>{Aii, M, sI, sI, (int)s2.size() - 1, 1, -1., 1.});
+        cs.push_back(T2.p + to);
+        cd.push_back(Aii + sI);
+        cm.push_back(r); cn.push_back(sI); cls.push_back(r); cld.push_back(M);
+        to += (size_t)r * sI;
+      }
+      gemm_batched<T>(t1, s1, s);
+      gemm_batched<T>(t2, s2, s);
+      copy2d_batched<T>(cs, cd, cm, cn, cls, cld, s);
+    }
+    // 5: off-diagonal rotations, in chunks of pairs
+    {
+      std::vector<long long> pairs;
+      for (long long i = 0; i < nb; i++)
+        for (long long ij = H.nrp[i]; ij < H.nrp[i + 1]; ij++)
+          if (ij != H.diag[i]) pairs.push_back(ij);
+      std::vector<long long> rowof(H.nrp.back());
+      for (long long i = 0; i < nb; i++)
+        for (long long ij = H.nrp[i]; ij < H.nrp[i + 1]; ij++) rowof[ij] = i;
+      size_t p0 = 0;
+      while (p0 < pairs.size()) {
+        size_t p1 = p0, need = 0;
+        std::vector<size_t> off;
+        while (p1 < pairs.size()) {
+          const long long ij = pairs[p1], i = rowof[ij], j = H.ncol[ij];
+          const size_t e = (size_t)S_(i) * M_(j) + (size_t)M_(i) * S_(j);
+          if (p1 > p0 && need + e > PAIR_BUDGET) break;
+          off.push_back(need);
+          need += e;
+          p1++;
+        }
+        DevBuf<T> P;
+        P.alloc(std::max<size_t>(need, 1));
+        std::vector<GemmTask<T>> ta, tb;
+        std::vector<GemmSeg<T>> sa, sb;
+        for (size_t q = p0; q < p1; q++) {
+          const long long ij = pairs[q], i = rowof[ij], j = H.ncol[ij];
+          const int M = M_(i), N = M_(j), si = S_(i), sj = S_(j), ri = M - si;
+          T* Pq = P.p + off[q - p0];             // s_i x N
+          T* Qq = Pq + (size_t)si * N;           // M x s_j
+          if (si > 0) {
+            sa.push_back(GemmSeg<T>{Ui(i), Aij(ij), M, M, M, 1, 0, 0, 0});
+            ta.push_back(GemmTask<T>{Pq, si, si, N, (int)sa.size() - 1, 1, 1., 0.});
+            sb.push_back(GemmSeg<T>{Pq, Ui(j), si, N, N, 0, 0, 0, 0});
+            tb.push_back(GemmTask<T>{Aij(ij), M, si, N, (int)sb.size() - 1, 1, 1., 0.});
+          }
+          if (sj > 0 && ri > 0) {
+            sa.push_back(GemmSeg<T>{Aij(ij), Ui(j), M, N, N, 0, 0, 0, 0});
+            ta.push_back(GemmTask<T>{Qq, M, M, sj, (int)sa.size() - 1, 1, 1., 0.});
+            sb.push_back(GemmSeg<T>{Vi(i) + si, Qq, M, M, M, 0, 0, 0, 0});
+            tb.push_back(GemmTask<T>{Aij(ij) + si, M, ri, sj, (int)sb.size() - 1, 1, 1., 0.});
+          }
+        }
+        gemm_batched<T>(ta, sa, s);
+        gemm_batched<T>(tb, sb, s);
+        p0 = p1;
+      }
+    }
+    // 6: Schur complements onto the skeleton
+    {
+      size_t t3 = 0;
+      for (long long i = 0; i < nb; i++)
+        for (long long ij = H.nrp[i]; ij < H.nrp[i + 1]; ij++)
+          if (ij != H.diag[i]) t3 += (size_t)S_(i) * (M_(H.ncol[ij]) - S_(H.ncol[ij]));
+      DevBuf<T> T3;
+      T3.alloc(std::max<size_t>(t3, 1));
+      std::vector<GemmTask<T>> ta, tb;
+      std::vector<GemmSeg<T>> sa, sb;
+      std::vector<size_t> t3off(H.nrp.back(), 0);
+      size_t to = 0;
+      for (long long i = 0; i < nb; i++) {
+        const int M = M_(i), si = S_(i), ri = M - si;
+        T* Aii = Aij(H.diag[i]);
+        for (long long ij = H.nrp[i]; ij < H.nrp[i + 1]; ij++) {
+          if (ij == H.diag[i]) continue;
+          const long long j = H.ncol[ij];
+          const int N = M_(j), sj = S_(j), rj = N - sj;
+          T* Aq = Aij(ij);
+          if (si > 0 && sj > 0) {
+            const int seg0 = (int)sa.size();
+            if (ri > 0) sa.push_back(GemmSeg<T>{Aii + (long long)si * M, Aq + si, M, M, ri, 0, 0, 0, 0});
+            if (rj > 0) sa.push_back(GemmSeg<T>{Aq + (long long)sj * M, Aij(H.diag[j]) + sj, M, N, rj, 0, 0, 0, 0});
+            ta.push_back(GemmTask<T>{Aq, M, si, sj, seg0, (int)sa.size() - seg0, -1., 1.});
+          }
+          if (si > 0 && rj > 0) {
+            t3off[ij] = to;
+            sa.push_back(GemmSeg<T>{Aq + (long long)sj * M, RRi(j), M, rj, rj, 0, 0, 0, 0});
+            ta.push_back(GemmTask<T>{T3.p + to, si, si, rj, (int)sa.size() - 1, 1, 1., 0.});
+            to += (size_t)si * rj;
+          }
+        }
+      }
+      for (long long i = 0; i < nb; i++) {
+        const int M = M_(i), si = S_(i);
+        if (si == 0) continue;
+        const int seg0 = (int)sb.size();
+        for (long long ij = H.nrp[i]; ij < H.nrp[i + 1]; ij++) {
+          if (ij == H.diag[i]) continue;
+          const long long j = H.ncol[ij];
+          const int sj = S_(j), rj = M_(j) - sj;
+          if (rj == 0) continue;
+          sb.push_back(GemmSeg<T>{T3.p + t3off[ij], Aij(ij) + (long long)sj * M, si, M, rj, 0, 1, 0, 0});
+        }
+        if ((int)sb.size() > seg0) tb.push_back(GemmTask<T>{Aij(H.diag[i]), M, si, si, seg0, (int)sb.size() - seg0, -1., 1.});
+      }
+      gemm_batched<T>(ta, sa, s);
+      gemm_batched<T>(tb, sb, s);
+    }
+  }
+
+  void copy_next(long long l) {
+    Level& H = h->lv[l];
+    Lv& F = lv[l];
+    Lv& P = lv[l - 1];
+    std::vector<const T*> cs;
+    std::vector<T*> cd;
+    std::vector<int> cm, cn, cls, cld;
+    for (long long i = 0; i < H.nb; i++)
+      for (long long ij = H.nrp[i]; ij < H.nrp[i + 1]; ij++) {
+        const long long j = H.ncol[ij];
+        const int si = (int)H.ranks[i], sj = (int)H.ranks[j];
+        if (si == 0 || sj == 0 || H.up_off[ij] < 0) continue;
+        cs.push_back(F.A.p + H.offA[ij]);
+        cd.push_back(P.A.p + H.up_off[ij]);
+        cm.push_back(si); cn.push_back(sj); cls.push_back((int)H.dims[i]); cld.push_back(H.up_ld[ij]);
+      }
+    copy2d_batched<T>(cs, cd, cm, cn, cls, cld, h->ctx->stream);
+  }
+
+  void solve(double* x) override {
+    hipStream_t s = h->ctx->stream;
+    const long long L = h->L;
+    T* v = vin.p;
+    if constexpr (sizeof(T) == 8)
+      MXG_HIP(hipMemcpyAsync(v, x, h->n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    else
+      convert(v, x, h->n, s);
+    for (long long l = L; l >= 0; l--) {
+      Level& H = h->lv[l];
+      Lv& F = lv[l];
+      const T* y = (l == L) ? v : lv[l + 1].Z.p;
+      hipLaunchKernelGGL(k_fwd1<T>, dim3((unsigned)H.nb), dim3(256), 0, s, H.d_dims.p, H.d_offQ.p, H.d_offX.p, F.V.p, y, F.X.p);
+      MXG_HIP(hipGetLastError());
+      if (H.lenZ > 0) {
+        hipLaunchKernelGGL(k_fwd2<T>, dim3((unsigned)H.nb), dim3(256), 0, s, H.d_dims.p, H.d_ranks.p, H.d_offX.p, H.d_offZ.p,
+                           H.d_nrp.p, H.d_ncol.p, H.d_offA.p, F.A.p, F.X.p, F.Z.p);
+        MXG_HIP(hipGetLastError());
+      }
+    }
+    for (long long l = 0; l <= L; l++) {
+      Level& H = h->lv[l];
+      Lv& F = lv[l];
+      // w of level l is the output of level l - 1 (stored in this level's Z buffer)
+      T* out = (l == L) ? v : lv[l + 1].Z.p;
+      size_t shm = std::max<long long>(H.maxM, 1) * sizeof(T);
+      hipLaunchKernelGGL(k_bwd<T>, dim3((unsigned)H.nb), dim3(256), shm, s, H.d_dims.p, H.d_ranks.p, H.d_offQ.p, H.d_offX.p,
+                         H.d_offZ.p, H.d_nrp.p, H.d_ncol.p, H.d_offA.p, F.A.p, F.U.p, F.X.p, F.Z.p, out);
+      MXG_HIP(hipGetLastError());
+    }
+    if constexpr (sizeof(T) == 8)
+      MXG_HIP(hipMemcpyAsync(x, v, h->n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    else
+      convert(x, v, h->n, s);
+  }
+
+  void info(int* out) override {
+    for (long long l = 0; l <= h->L; l++) out[l] = lv[l].info_sum;
+  }
+};
+
+}  // namespace mxg
+
+struct mxg_ulv_s {
+  mxg_h2_s* h = nullptr;
+  int precision = MXG_FP64;
+  double seconds = 0.;
+  std::unique_ptr<mxg::UlvBase> impl;
+};
+
+using namespace mxg;
+
+extern "C" {
+
+int mxg_ulv_create(mxg_h2_t h, int precision, mxg_ulv_t* out) {
+  return guarded([&] {
+    MXG_REQUIRE(h && out, "null argument");
+    for (long long l = 0; l <= h->L; l++) MXG_REQUIRE(h->lv[l].built, "H2 matrix not fully constructed");
+    MXG_REQUIRE(h->flags & MXG_H2_FACTORABLE, "H2 matrix was created without MXG_H2_FACTORABLE");
+    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32, "unknown precision");
+    MXG_HIP(hipSetDevice(h->ctx->device));
+    auto f = std::make_unique<mxg_ulv_s>();
+    f->h = h;
+    f->precision = precision;
+    auto t0 = std::chrono::steady_clock::now();
+    if (precision == MXG_FP64)
+      f->impl = std::make_unique<UlvImpl<double>>(h);
+    else
+      f->impl = std::make_unique<UlvImpl<float>>(h);
+    f->impl->factorize();
+    MXG_HIP(hipStreamSynchronize(h->ctx->stream));
+    f->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    *out = f.release();
+  });
+}
+
+int mxg_ulv_destroy(mxg_ulv_t f) {
+  return guarded([&] {
+    if (!f) return;
+    (void)hipStreamSynchronize(f->h->ctx->stream);
+    delete f;
+  });
+}
+
+int mxg_ulv_info(mxg_ulv_t f, int* info_per_level) {
+  return guarded([&] {
+    MXG_REQUIRE(f && info_per_level, "null argument");
+    f->impl->info(info_per_level);
+  });
+}
+
+int mxg_ulv_solve(mxg_ulv_t f, double* d_x, long long nrhs, long long ld) {
+  return guarded([&] {
+    MXG_REQUIRE(f && d_x, "null argument");
+    MXG_REQUIRE(nrhs >= 1 && ld >= f->h->n, "bad nrhs/ld");
+    MXG_HIP(hipSetDevice(f->h->ctx->device));
+    for (long long c = 0; c < nrhs; c++) f->impl->solve(d_x + c * ld);
+    MXG_HIP(hipStreamSynchronize(f->h->ctx->stream));
+  });
+}
+
+double mxg_ulv_factor_seconds(mxg_ulv_t f) { return f ? f->seconds : -1.; }
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+"""Python mirror of the host API (H2MatrixSolver, solver.hpp:11-47), driving
+the C++ host layer (libmxc_h2) and through it the HIP device layer.
+
+Vectors are DeviceArray objects holding tree-ordered FP64 data.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .device import Context, DeviceArray, kernel_desc
+
+GEOMETRY = {"cube": 0, "sphere": 1, "mesh": 2}
+
+
+def generate_points(geometry: str, n: int, seed: int) -> np.ndarray:
+    out = np.zeros(3 * n)
+    N.check_h2(N.h2().mxc_generate_points(GEOMETRY[geometry], n, seed, out.ctypes.data))
+    return out
+
+
+def uniform_vector(n: int, lo: float, hi: float, seed: int) -> np.ndarray:
+    x = np.zeros(n)
+    N.check_h2(N.h2().mxc_uniform_vector(x.ctypes.data, n, lo, hi, seed))
+    return x
+
+
+def tree_levels(n: int, leaf: int) -> int:
+    return int(N.h2().mxc_tree_levels(n, leaf))
+
+
+def build_tree(points: np.ndarray, levels: int):
+    """Tree-ordered copy of the points and the cell arrays (child, body, R, C)."""
+    b = np.ascontiguousarray(points, dtype=np.float64).copy()
+    nc = (2 << levels) - 1
+    child = np.zeros(2 * nc, np.int64)
+    body = np.zeros(2 * nc, np.int64)
+    R = np.zeros(3 * nc)
+    Cc = np.zeros(3 * nc)
+    N.check_h2(N.h2().mxc_build_tree(b.ctypes.data, len(b) // 3, levels, child.ctypes.data, body.ctypes.data,
+                                     R.ctypes.data, Cc.ctypes.data))
+    return b, child, body, R, Cc
+
+
+def csr(far: bool, theta: float, levels: int, child, body, R, Cc):
+    nc = (2 << levels) - 1
+    rp = np.zeros(nc + 1, np.int64)
+    L = N.h2()
+    nnz = L.mxc_csr(int(far), theta, levels, child.ctypes.data, body.ctypes.data, R.ctypes.data, Cc.ctypes.data,
+                    rp.ctypes.data, None, 0)
+    if nnz < 0:
+        raise N.NativeError(L.mxc_last_error().decode())
+    col = np.zeros(max(nnz, 1), np.int64)
+    L.mxc_csr(int(far), theta, levels, child.ctypes.data, body.ctypes.data, R.ctypes.data, Cc.ctypes.data,
+              rp.ctypes.data, col.ctypes.data, nnz)
+    return rp, col[:nnz]
+
+
+def far_sample(levels: int, body, far_rp, far_col, cell: int, sampling: int) -> np.ndarray:
+    L = N.h2()
+    n = L.mxc_far_sample(levels, body.ctypes.data, far_rp.ctypes.data, far_col.ctypes.data, cell, sampling, None, 0)
+    out = np.zeros(max(n, 1), np.int64)
+    L.mxc_far_sample(levels, body.ctypes.data, far_rp.ctypes.data, far_col.ctypes.data, cell, sampling,
+                     out.ctypes.data, n)
+    return out[:n]
+
+
+class Problem:
+    """Points sorted into tree order and resident on one device."""
+
+    def __init__(self, ctx: Context, points: np.ndarray, leaf: int):
+        self.ctx = ctx
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        self.n = len(pts) // 3
+        h = C.c_void_p()
+        N.check_h2(N.h2().mxc_problem_create(ctx.handle, pts.ctypes.data, self.n, leaf, C.byref(h)))
+        self.handle = h.value
+        self.levels = int(N.h2().mxc_problem_levels(self.handle))
+        self.bodies = np.zeros(3 * self.n)
+        N.check_h2(N.h2().mxc_problem_bodies(self.handle, self.bodies.ctypes.data))
+
+    def dense_matvec(self, kernel: str, diag: float, alpha: float, x: DeviceArray, y: DeviceArray) -> None:
+        N.check_h2(N.h2().mxc_dense_matvec(self.handle, kernel_desc(kernel, diag, alpha), x.ptr, y.ptr))
+
+    def close(self):
+        if self.handle:
+            N.h2().mxc_problem_destroy(self.handle)
+            self.handle = None
+
+
+class H2:
+    """An H^2 matrix on the device (H2MatrixSolver)."""
+
+    def __init__(self, problem: Problem, kernel="laplace", diag=1e3, alpha=1.0, theta=2.0, tol=0.0, rank=0,
+                 leveled_rank=0, orth=False, hss_far=False, sampling=0, factorable=False):
+        self.problem = problem
+        self.kernel = (kernel, diag, alpha)
+        o = N.MxcH2Opts(N.KERNELS[kernel], diag, alpha, theta, tol, rank, leveled_rank, int(orth), int(hss_far), sampling,
+                        int(factorable))
+        h = C.c_void_p()
+        N.check_h2(N.h2().mxc_solver_create(problem.handle, C.byref(o), C.byref(h)))
+        self.handle = h.value
+        self.levels = problem.levels
+
+    @classmethod
+    def operator(cls, problem, eps, sampling, kernel="laplace", diag=1e3, alpha=1.0, theta=2.0):
+        """The accurate operator of lorasp: interpolative bases at tolerance eps (solver.cpp:28-36, epi < 1)."""
+        return cls(problem, kernel, diag, alpha, theta, tol=eps, sampling=sampling)
+
+    @classmethod
+    def preconditioner(cls, problem, rank, sampling, kernel="laplace", diag=1e3, alpha=1.0, theta=2.0, eps=0.0,
+                       leveled_rank=0):
+        """The ULV preconditioner of lorasp: orthogonal bases of fixed rank (epi == 0), or truncated at eps
+        with the rank as a cap, basis far lists from the sibling lists (solver.cpp:28-36)."""
+        return cls(problem, kernel, diag, alpha, theta, tol=eps, rank=rank, leveled_rank=leveled_rank, orth=True,
+                   hss_far=True, sampling=sampling, factorable=True)
+
+    def dims(self, level: int):
+        nb = 1 << level
+        d = np.zeros(nb, np.int64)
+        r = np.zeros(nb, np.int64)
+        N.check_h2(N.h2().mxc_solver_dims(self.handle, level, d.ctypes.data, r.ctypes.data))
+        return d, r
+
+    def counts(self, level: int):
+        c = np.zeros(2, np.int64)
+        N.check_h2(N.h2().mxc_solver_counts(self.handle, level, c.ctypes.data))
+        return int(c[0]), int(c[1])
+
+    def matvec(self, x: DeviceArray, y: DeviceArray, precision: int = N.MXG_FP64) -> None:
+        N.check_h2(N.h2().mxc_solver_matvec(self.handle, x.ptr, y.ptr, precision))
+
+    def factorize(self, precision: int = N.MXG_FP64) -> np.ndarray:
+        info = np.zeros(self.levels + 1, np.int32)
+        N.check_h2(N.h2().mxc_solver_factorize(self.handle, precision, info.ctypes.data))
+        return info
+
+    def factor_seconds(self) -> float:
+        return float(N.gpu().mxg_ulv_factor_seconds(N.h2().mxc_solver_ulv(self.handle)))
+
+    def precondition(self, x: DeviceArray) -> None:
+        N.check_h2(N.h2().mxc_solver_precondition(self.handle, x.ptr))
+
+    def gmres(self, M: "H2", b: DeviceArray, x: DeviceArray, tol: float, restart: int = 10, max_cycles: int = 50,
+              mode: int = N.GMRES_REFERENCE, inner_precision: int = N.MXG_FP64):
+        o = N.MxgGmresOpts(tol, restart, max_cycles, mode, inner_precision)
+        resid = np.zeros(max_cycles + 1)
+        it = C.c_longlong()
+        inner = C.c_longlong()
+        N.check_h2(N.h2().mxc_solver_gmres(self.handle, M.handle, C.byref(o), b.ptr, x.ptr, resid.ctypes.data,
+                                           C.byref(it), C.byref(inner)))
+        return resid[: it.value + 1], it.value, inner.value
+
+    def close(self):
+        if self.handle:
+            N.h2().mxc_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
