@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Solves/s of the H^2 solve path on one MI355X.
+
+Workload (the metric's configuration): N = 2^20 points uniform on the unit
+sphere (seed 2), Yukawa kernel exp(-|x-y|)/|x-y| with K(0) = 1e3, theta = 2,
+leaf 256. The operator is an interpolative H^2 matrix at tolerance OP_EPS
+(far-body sampling 2000 per box); the preconditioner is an orthogonal H^2
+matrix truncated at 1e-6 with rank cap 64, ULV-factorized. A step is one
+solve: GMRES from b (U(-1, 1), a fresh seed per step) to
+||b - A x|| / ||b|| <= 1e-10 on the FP64 H^2 operator, restart 30, with the
+factorization resident in HBM. The mixed path runs the ULV in FP32 and the
+inner operator applications in FP32 (FP64 Arnoldi, Givens, outer residual);
+the FP64 path runs everything in FP64. `value` is the mixed path's solves/s;
+the FP64 path is reported beside it. After the timed region every solution is
+checked with the matrix-free FP64 dense matvec.
+
+Prints one JSON line (see the keys below).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20, help="points (default 2^20)")
+    ap.add_argument("--op-eps", type=float, default=1e-12)
+    ap.add_argument("--sampling", type=int, default=2000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=16384, help="points of the CPU baseline sample")
+    return ap.parse_args()
+
+
+def pct(a, q):
+    return float(np.percentile(np.asarray(a), q))
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build targets one GPU (the H^2 solve path keeps the whole tree on one device); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    from mxc_amd import _native as N
+    from mxc_amd.device import Context, kernel_desc
+    from mxc_amd.solver import H2, Problem, generate_points, uniform_vector
+
+    n, leaf, theta = args.n, 256, 2.0
+    kernel, diag, alpha = "yukawa", 1e3, 1.0
+    tol, restart, max_cycles = 1e-10, 30, 20
+    ctx = Context(0)
+    t0 = time.time()
+    pts = generate_points("sphere", n, 2)
+    prob = Problem(ctx, pts, leaf)
+    A = H2.operator(prob, args.op_eps, args.sampling, kernel, diag, alpha, theta)
+    t_op = time.time() - t0
+    t0 = time.time()
+    M = H2.preconditioner(prob, 64, args.sampling, kernel, diag, alpha, theta, eps=1e-6)
+    t_pc = time.time() - t0
+    L = prob.levels
+
+    rhs = [ctx.array(uniform_vector(n, -1.0, 1.0, 1000 + k)) for k in range(args.warmup + args.steps)]
+    xs = [ctx.zeros(n) for _ in range(args.steps)]
+    work = ctx.zeros(n)
+
+    def solve(b, x, inner_prec):
+        x.zero()
+        res, it, inner = A.gmres(M, b, x, tol, restart, max_cycles, N.GMRES_CGS2, inner_prec)
+        return res[-1], it, inner
+
+    def run_path(factor_prec, inner_prec, prof_class):
+        info = M.factorize(factor_prec)
+        assert not info.any(), f"singular ULV blocks per level: {info}"
+        fsec = M.factor_seconds()
+        for k in range(args.warmup):
+            solve(rhs[k], work, inner_prec)
+        ctx.synchronize()
+        N.check_gpu(N.gpu().mxg_profile_reset(ctx.handle))
+        N.check_gpu(N.gpu().mxg_profile_enable(ctx.handle, 1))
+        lat, cycles, inners, res = [], [], [], []
+        ctx.synchronize()
+        t_start = time.perf_counter()
+        for k in range(args.steps):
+            ts = time.perf_counter()
+            r, it, inner = solve(rhs[args.warmup + k], xs[k], inner_prec)
+            lat.append(time.perf_counter() - ts)
+            res.append(r)
+            cycles.append(it)
+            inners.append(inner)
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t_start
+        N.check_gpu(N.gpu().mxg_profile_enable(ctx.handle, 0))
+        kern = {}
+        import ctypes as C
+        for cls, name in ((0, "matvec_leaf_fp64"), (1, "matvec_leaf_fp32"), (2, "ulv_fwd_leaf"), (3, "ulv_bwd_leaf"),
+                          (4, "matvec_upper")):
+            cnt, ms = C.c_longlong(), C.c_double()
+            N.check_gpu(N.gpu().mxg_profile_read(ctx.handle, cls, C.byref(cnt), C.byref(ms)))
+            kern[name] = {"launches": cnt.value, "total_ms": ms.value,
+                          "avg_ms": ms.value / cnt.value if cnt.value else None}
+        # dense FP64 check of every solution (outside the timed region)
+        dense = []
+        for k in range(args.steps):
+            prob.dense_matvec(kernel, diag, alpha, xs[k], work)
+            ax = work.download()
+            b = rhs[args.warmup + k].download()
+            dense.append(float(np.linalg.norm(b - ax) / np.linalg.norm(b)))
+        return dict(elapsed=elapsed, lat=lat, cycles=cycles, inner=inners, h2_resid=res, dense=dense, factor_s=fsec,
+                    kernels=kern)
+
+    fp64 = run_path(N.MXG_FP64, N.MXG_FP64, 0)
+    mixed = run_path(N.MXG_FP32, N.MXG_FP32, 1)
+
+    # roofline of the dominant kernel of the mixed path: the FP32 leaf matvec
+    h = N.h2().mxc_solver_device(A.handle)
+    d, r = A.dims(L)
+    near_elems = int(N.gpu().mxg_h2_level_size(h, L, 0))
+    c_elems = int(N.gpu().mxg_h2_level_size(h, L, 3))
+    q_elems = int(np.sum(d * r))
+    lenz = int(np.sum(r))
+    bytes_leaf_fp32 = 4 * (near_elems + c_elems + q_elems + 2 * n + 2 * lenz)
+    km = mixed["kernels"]["matvec_leaf_fp32"]
+    achieved = bytes_leaf_fp32 / (km["avg_ms"] * 1e-3) / 1e9 if km["avg_ms"] else None
+
+    value = args.steps / mixed["elapsed"]
+    value64 = args.steps / fp64["elapsed"]
+    out = {
+        "metric": "solves/s at rel-res 1e-10, mixed vs FP64 ULV, N=2^20; p50/p99 solve latency",
+        "value": round(value, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * mixed["elapsed"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 ULV + f32 inner operator, f64 Arnoldi/residual",
+        "data": "synthetic (seeded points and right-hand sides)",
+        "config": {
+            "workload": "H2 GMRES solve, sphere Yukawa N=2^20 (configs[2])",
+            "N": n, "leaf": leaf, "theta": theta, "kernel": "yukawa exp(-r)/r, K(0)=1e3",
+            "operator_eps": args.op_eps, "sampling": args.sampling, "precond_eps": 1e-6, "precond_max_rank": 64,
+            "restart": restart, "tol": tol, "levels": L,
+        },
+        "p50_ms": round(1e3 * pct(mixed["lat"], 50), 3),
+        "p99_ms": round(1e3 * pct(mixed["lat"], 99), 3),
+        "fp64_path": {
+            "solves_per_s": round(value64, 4), "p50_ms": round(1e3 * pct(fp64["lat"], 50), 3),
+            "p99_ms": round(1e3 * pct(fp64["lat"], 99), 3), "cycles": fp64["cycles"], "inner_iters": fp64["inner"],
+            "dense_relres_max": max(fp64["dense"]), "factor_s": round(fp64["factor_s"], 4),
+        },
+        "mixed_over_fp64": round(value / value64, 3),
+        "mixed_path": {
+            "cycles": mixed["cycles"], "inner_iters": mixed["inner"], "h2_relres_max": max(mixed["h2_resid"]),
+            "dense_relres_max": max(mixed["dense"]), "factor_s": round(mixed["factor_s"], 4),
+        },
+        "setup_s": {"operator": round(t_op, 2), "preconditioner": round(t_pc, 2)},
+        "kernels_mixed": mixed["kernels"],
+        "roofline": {
+            "kernel": "k_down<float, leaf> (FP32 leaf matvec: near field + couplings + basis)",
+            "bound": "hbm",
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "algorithmic_bytes_per_launch": bytes_leaf_fp32,
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.sampling, args.op_eps, n)
+    print(json.dumps(out))
+
+
+def cpu_baseline(ns, sampling, op_eps, n_target):
+    """The CPU oracle (FP64 restatement of the reference) on the same workload
+    at ns points: construction and factorization untimed, then FP64 GMRES
+    solves (restart 30, tol 1e-10) timed on the host cores; solves/s scaled to
+    N = n_target by the O(N) cost of a solve."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, 16))
+    pts = O.sphere_points(ns, 2)
+    P = O.Problem(pts, 256)
+    A = P.h2(kernel="yukawa", alpha=1.0, tol=op_eps, sampling=sampling, nthreads=threads)
+    M = P.h2(kernel="yukawa", alpha=1.0, tol=1e-6, rank=64, orth=True, hss_far=True, sampling=sampling, nthreads=threads)
+    M.factorize()
+    times = []
+    t_all = time.perf_counter()
+    k = 0
+    while True:
+        b = O.uniform_vec(ns, -1, 1, 1000 + k)
+        ts = time.perf_counter()
+        x, res = A.gmres(M, b, 1e-10, 30, 20)
+        times.append(time.perf_counter() - ts)
+        k += 1
+        if time.perf_counter() - t_all > 10.0 or k >= 20:
+            break
+    per = float(np.mean(times))
+    return {
+        "value": round((1.0 / per) * ns / n_target, 6),
+        "unit": "solves/s (scaled to N=2^20)",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle GMRES solves (reference mode, CGS1, restart 30) at N={ns} sphere Yukawa, "
+                  f"{k} solves, {per * 1e3:.1f} ms/solve measured, scaled by {ns}/{n_target}",
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -35,10 +35,14 @@ int mxc_build_tree(double* bodies, long long n, long long levels, long long* chi
  * (ncells + 1) and col when cap >= nnz. */
 long long mxc_csr(int far, double theta, long long levels, const long long* child, const long long* body,
                   const double* R, const double* C, long long* rowptr, long long* col, long long cap);
-/* Sampled far-body list of one cell given the far CSR; returns its length and
- * fills out when cap >= length. */
-long long mxc_far_sample(long long levels, const long long* body, const long long* far_rp, const long long* far_col,
-                         long long cell, long long sampling, long long* out, long long cap);
+/* Sampled far-body list of one cell: bfar = the basis far lists (Far, or the
+ * theta = 0 sibling lists), near/far = the matrix's theta lists defining the
+ * distance shells, with_near != 0 when bfar covers every body outside the
+ * cell. Returns the length and fills out when cap >= length. */
+long long mxc_far_sample(long long levels, const long long* body, const long long* bfar_rp, const long long* bfar_col,
+                         const long long* near_rp, const long long* near_col, const long long* far_rp,
+                         const long long* far_col, int with_near, long long cell, long long sampling, long long* out,
+                         long long cap);
 
 /* ---- problem: points + tree on one device ---- */
 typedef struct mxc_problem_s* mxc_problem_t;

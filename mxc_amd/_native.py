@@ -89,6 +89,9 @@ def gpu() -> C.CDLL:
             "mxg_ulv_solve": [P, P, c_ll, c_ll],
             "mxg_ulv_info": [P, P],
             "mxg_gmres": [P, P, C.POINTER(MxgGmresOpts), P, P, P, C.POINTER(c_ll), C.POINTER(c_ll)],
+            "mxg_profile_enable": [P, c_int],
+            "mxg_profile_reset": [P],
+            "mxg_profile_read": [P, c_int, C.POINTER(c_ll), C.POINTER(c_d)],
         }
         for name, args in sigs.items():
             fn = getattr(L, name, None)
@@ -146,7 +149,7 @@ def h2() -> C.CDLL:
         L.mxc_solver_ulv.restype = P
         L.mxc_csr.argtypes = [c_int, c_d, c_ll, P, P, P, P, P, P, c_ll]
         L.mxc_csr.restype = c_ll
-        L.mxc_far_sample.argtypes = [c_ll, P, P, P, c_ll, c_ll, P, c_ll]
+        L.mxc_far_sample.argtypes = [c_ll, P, P, P, P, P, P, P, c_int, c_ll, c_ll, P, c_ll]
         L.mxc_far_sample.restype = c_ll
         _h2 = L
     return _h2

@@ -142,4 +142,25 @@ __device__ inline int find_task(const long long* prefix, int ntasks, long long x
 struct mxg_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  // kernel timing: while enabled, instrumented launches are bracketed by HIP
+  // events on the context stream (one start/stop pair per launch)
+  bool profiling = false;
+  struct Timed {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    long long launches = 0;
+    double ms = 0.;
+  };
+  Timed timed[MXG_PROF_CLASSES];
+  std::vector<hipEvent_t> event_pool;
 };
+
+namespace mxg {
+// Brackets one launch of kernel class `cls` with events when profiling.
+struct ProfScope {
+  mxg_ctx_s* c;
+  int cls;
+  hipEvent_t stop = nullptr;
+  ProfScope(mxg_ctx_s* ctx, int k);
+  ~ProfScope();
+};
+}  // namespace mxg

@@ -8,6 +8,46 @@ namespace {
 thread_local std::string g_last_error;
 }
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+hipEvent_t take_event(mxg_ctx_s* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  MXG_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void drain(mxg_ctx_s* c) {
+  MXG_HIP(hipStreamSynchronize(c->stream));
+  for (auto& t : c->timed) {
+    for (auto& pr : t.pending) {
+      float ms = 0.f;
+      MXG_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+      t.ms += ms;
+      t.launches++;
+      c->event_pool.push_back(pr.first);
+      c->event_pool.push_back(pr.second);
+    }
+    t.pending.clear();
+  }
+}
+}  // namespace
+
+ProfScope::ProfScope(mxg_ctx_s* ctx, int k) : c(ctx), cls(k) {
+  if (!c->profiling || cls < 0 || cls >= MXG_PROF_CLASSES) return;
+  hipEvent_t start = take_event(c);
+  stop = take_event(c);
+  MXG_HIP(hipEventRecord(start, c->stream));
+  c->timed[cls].pending.emplace_back(start, stop);
+}
+
+ProfScope::~ProfScope() {
+  if (stop) (void)hipEventRecord(stop, c->stream);
+}
 }  // namespace mxg
 
 using namespace mxg;
@@ -42,10 +82,40 @@ int mxg_init(int device, mxg_ctx_t* ctx) {
   });
 }
 
+int mxg_profile_enable(mxg_ctx_t ctx, int enable) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    ctx->profiling = enable != 0;
+  });
+}
+
+int mxg_profile_reset(mxg_ctx_t ctx) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    drain(ctx);
+    for (auto& t : ctx->timed) {
+      t.launches = 0;
+      t.ms = 0.;
+    }
+  });
+}
+
+int mxg_profile_read(mxg_ctx_t ctx, int cls, long long* launches, double* total_ms) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx && launches && total_ms, "null argument");
+    MXG_REQUIRE(cls >= 0 && cls < MXG_PROF_CLASSES, "kernel class out of range");
+    drain(ctx);
+    *launches = ctx->timed[cls].launches;
+    *total_ms = ctx->timed[cls].ms;
+  });
+}
+
 int mxg_finalize(mxg_ctx_t ctx) {
   return guarded([&] {
     if (!ctx) return;
     MXG_HIP(hipSetDevice(ctx->device));
+    drain(ctx);
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     MXG_HIP(hipStreamSynchronize(ctx->stream));
     MXG_HIP(hipStreamDestroy(ctx->stream));
     delete ctx;

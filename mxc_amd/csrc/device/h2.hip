@@ -477,6 +477,7 @@ void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y) {
     Level& Lv = h->lv[l];
     const T* xin = (l == L) ? x : Zp(l + 1);
     if (Lv.lenZ == 0) continue;
+    ProfScope prof(h->ctx, MXG_PROF_MATVEC_UPPER);
     hipLaunchKernelGGL(k_up<T>, dim3((unsigned)Lv.nb), dim3(256), 0, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p, Lv.d_offX.p,
                        Lv.d_offZ.p, Qp(Lv), xin, Zp(l));
     MXG_HIP(hipGetLastError());
@@ -486,6 +487,8 @@ void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y) {
     const bool leaf = (l == L);
     T* out = leaf ? y : Wp(l + 1);
     size_t shm = std::max<long long>(Lv.maxK, 1) * sizeof(T);
+    ProfScope prof(h->ctx, leaf ? (sizeof(T) == 8 ? MXG_PROF_MATVEC_LEAF_FP64 : MXG_PROF_MATVEC_LEAF_FP32)
+                                : MXG_PROF_MATVEC_UPPER);
     if (leaf)
       hipLaunchKernelGGL((k_down<T, true>), dim3((unsigned)Lv.nb), dim3(256), shm, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p,
                          Lv.d_offX.p, Lv.d_offZ.p, Lv.d_frp.p, Lv.d_fcol.p, Lv.d_offC.p, Lv.d_nrp.p, Lv.d_ncol.p, Lv.d_offA.p,

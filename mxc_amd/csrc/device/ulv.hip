@@ -406,6 +406,7 @@ struct UlvImpl : UlvBase {
       Level& H = h->lv[l];
       Lv& F = lv[l];
       const T* y = (l == L) ? v : lv[l + 1].Z.p;
+      ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_FWD_LEAF : -1);
       hipLaunchKernelGGL(k_fwd1<T>, dim3((unsigned)H.nb), dim3(256), 0, s, H.d_dims.p, H.d_offQ.p, H.d_offX.p, F.V.p, y, F.X.p);
       MXG_HIP(hipGetLastError());
       if (H.lenZ > 0) {
@@ -420,6 +421,7 @@ struct UlvImpl : UlvBase {
       // w of level l is the output of level l - 1 (stored in this level's Z buffer)
       T* out = (l == L) ? v : lv[l + 1].Z.p;
       size_t shm = std::max<long long>(H.maxM, 1) * sizeof(T);
+      ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_BWD_LEAF : -1);
       hipLaunchKernelGGL(k_bwd<T>, dim3((unsigned)H.nb), dim3(256), shm, s, H.d_dims.p, H.d_ranks.p, H.d_offQ.p, H.d_offX.p,
                          H.d_offZ.p, H.d_nrp.p, H.d_ncol.p, H.d_offA.p, F.A.p, F.U.p, F.X.p, F.Z.p, out);
       MXG_HIP(hipGetLastError());

@@ -128,15 +128,21 @@ long long mxc_csr(int far, double theta, long long levels, const long long* chil
   }
 }
 
-long long mxc_far_sample(long long levels, const long long* body, const long long* far_rp, const long long* far_col,
-                         long long cell, long long sampling, long long* out, long long cap) {
+long long mxc_far_sample(long long levels, const long long* body, const long long* bfar_rp, const long long* bfar_col,
+                         const long long* near_rp, const long long* near_col, const long long* far_rp,
+                         const long long* far_col, int with_near, long long cell, long long sampling, long long* out,
+                         long long cap) {
   try {
     std::vector<mxc::Cell> cells = cells_from(levels, nullptr, body, nullptr, nullptr);
-    mxc::CSR f;
     const long long nc = (2ll << levels) - 1;
-    f.RowIndex.assign(far_rp, far_rp + nc + 1);
-    f.ColIndex.assign(far_col, far_col + far_rp[nc]);
-    std::vector<long long> idx = mxc::sampleFarBodies(cells, f, cell, sampling);
+    auto mk = [&](const long long* rp, const long long* col) {
+      mxc::CSR c;
+      c.RowIndex.assign(rp, rp + nc + 1);
+      c.ColIndex.assign(col, col + rp[nc]);
+      return c;
+    };
+    mxc::CSR bf = mk(bfar_rp, bfar_col), nr = mk(near_rp, near_col), fr = mk(far_rp, far_col);
+    std::vector<long long> idx = mxc::sampleFarBodies(cells, bf, nr, fr, with_near != 0, cell, sampling);
     if (cap >= (long long)idx.size()) std::copy(idx.begin(), idx.end(), out);
     return (long long)idx.size();
   } catch (const std::exception& e) {

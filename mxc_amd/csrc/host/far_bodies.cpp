@@ -6,50 +6,73 @@ namespace mxc {
 
 namespace {
 
-struct Segment {
-  std::vector<std::pair<long long, long long>> ranges;  // body ranges of the far cells
+struct Shell {
+  std::vector<std::pair<long long, long long>> ranges;  // body ranges of the shell's cells
   long long count = 0;
   long long quota = 0;
+  void add(const Cell& c) {
+    ranges.emplace_back(c.Body[0], c.Body[1]);
+    count += c.Body[1] - c.Body[0];
+  }
 };
 
-}  // namespace
-
-std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR& far, long long cell, long long sampling) {
-  // ancestors from level 1 down to the cell itself
+// ancestors of `cell` from level 1 down to the cell itself
+std::vector<long long> ancestry(long long cell) {
   std::vector<long long> path;
   for (long long c = cell; c > 0; c = (c - 1) >> 1) path.push_back(c);
   std::reverse(path.begin(), path.end());
+  return path;
+}
 
-  std::vector<Segment> seg(path.size());
+}  // namespace
+
+std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR& basis_far, const CSR& near,
+                                       const CSR& far, bool with_near, long long cell, long long sampling) {
+  const std::vector<long long> path = ancestry(cell);
   long long total = 0;
-  for (size_t a = 0; a < path.size(); a++) {
-    for (long long k = far.RowIndex[path[a]]; k < far.RowIndex[path[a] + 1]; k++) {
-      const Cell& f = cells[far.ColIndex[k]];
-      seg[a].ranges.emplace_back(f.Body[0], f.Body[1]);
-      seg[a].count += f.Body[1] - f.Body[0];
+  for (long long a : path)
+    for (long long k = basis_far.RowIndex[a]; k < basis_far.RowIndex[a + 1]; k++) {
+      const Cell& f = cells[basis_far.ColIndex[k]];
+      total += f.Body[1] - f.Body[0];
     }
-    seg[a].quota = seg[a].count;
-    total += seg[a].count;
-  }
-
-  if (sampling > 0 && total > sampling) {
-    long long left = sampling;
-    for (size_t a = seg.size(); a-- > 0;) {
-      long long cap = (a == 0) ? left : (left + 1) / 2;
-      seg[a].quota = std::min(seg[a].count, cap);
-      left -= seg[a].quota;
-    }
-    for (size_t a = seg.size(); a-- > 0 && left > 0;) {
-      long long more = std::min(seg[a].count - seg[a].quota, left);
-      seg[a].quota += more;
-      left -= more;
-    }
-  }
-
   std::vector<long long> out;
-  out.reserve(std::min(total, sampling > 0 ? sampling : total));
-  for (const Segment& s : seg) {
-    if (s.quota == 0) continue;
+  if (sampling <= 0 || total <= sampling) {
+    out.reserve(total);
+    for (long long a : path)
+      for (long long k = basis_far.RowIndex[a]; k < basis_far.RowIndex[a + 1]; k++) {
+        const Cell& f = cells[basis_far.ColIndex[k]];
+        for (long long b = f.Body[0]; b < f.Body[1]; b++) out.push_back(b);
+      }
+    return out;
+  }
+
+  // shells, closest first
+  std::vector<Shell> sh;
+  if (with_near) {
+    sh.emplace_back();
+    for (long long k = near.RowIndex[cell]; k < near.RowIndex[cell + 1]; k++)
+      if (near.ColIndex[k] != cell) sh.back().add(cells[near.ColIndex[k]]);
+  }
+  for (size_t a = path.size(); a-- > 0;) {
+    sh.emplace_back();
+    for (long long k = far.RowIndex[path[a]]; k < far.RowIndex[path[a] + 1]; k++) sh.back().add(cells[far.ColIndex[k]]);
+  }
+
+  long long left = sampling;
+  for (size_t k = 0; k < sh.size(); k++) {
+    long long cap = (k + 1 == sh.size()) ? left : (left + 1) / 2;
+    sh[k].quota = std::min(sh[k].count, cap);
+    left -= sh[k].quota;
+  }
+  for (size_t k = 0; k < sh.size() && left > 0; k++) {
+    long long more = std::min(sh[k].count - sh[k].quota, left);
+    sh[k].quota += more;
+    left -= more;
+  }
+
+  out.reserve(sampling);
+  for (size_t k = sh.size(); k-- > 0;) {
+    const Shell& s = sh[k];
     size_t r = 0;
     long long before = 0;  // bodies in ranges[0..r)
     for (long long t = 0; t < s.quota; t++) {
@@ -64,11 +87,12 @@ std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR
   return out;
 }
 
-FarBodies::FarBodies(const std::vector<Cell>& cells, const CSR& far, long long level, long long sampling) {
+FarBodies::FarBodies(const std::vector<Cell>& cells, const CSR& basis_far, const CSR& near, const CSR& far, bool with_near,
+                     long long level, long long sampling) {
   const long long nb = 1ll << level, first = nb - 1;
   std::vector<std::vector<long long>> lists(nb);
 #pragma omp parallel for schedule(dynamic, 16)
-  for (long long i = 0; i < nb; i++) lists[i] = sampleFarBodies(cells, far, first + i, sampling);
+  for (long long i = 0; i < nb; i++) lists[i] = sampleFarBodies(cells, basis_far, near, far, with_near, first + i, sampling);
   offsets.assign(nb + 1, 0);
   for (long long i = 0; i < nb; i++) offsets[i + 1] = offsets[i] + (long long)lists[i].size();
   indices.resize(offsets[nb]);

@@ -3,17 +3,24 @@
 // Replaces the only part of the reference's Hmatrix that reaches the results:
 // the per-box far-body list m (h-matrix.cpp:121-127,160-165,169-175), which is
 // the parent's list followed by the bodies of the box's own far-list cells in
-// CSR order. The reference's randomized SVDs (h-matrix.cpp:103-120) never
-// reach a result and are not built.
+// CSR order, built from the basis far lists (Far, or the sibling lists of
+// HSS_Far for the preconditioner, solver.cpp:32). The reference's randomized
+// SVDs (h-matrix.cpp:103-120) never reach a result and are not built.
 //
-// Sampling (the `sampling` argument of lorasp; the reference HEAD has no cap):
-// the list is a sequence of per-level segments, coarsest ancestor first. With
-// s = sampling < list length, quotas go from the box's own level upward, each
-// level taking at most half of the remaining budget (the coarsest takes the
-// rest); quota a segment cannot use is then handed back from the finest level
-// upward. A segment of n bodies with quota q keeps positions floor(t n / q),
-// t = 0..q-1. sampling <= 0, or a list no longer than s, keeps every body in
-// reference order.
+// Sampling (the `sampling` argument of lorasp; the reference HEAD has no cap).
+// When the list is longer than s = sampling, its bodies are grouped into
+// distance shells from the matrix's own near/far structure (theta): shell 0 =
+// the box's near cells other than itself (only when the list holds every
+// body outside the box, as with the sibling lists), shell 1 = the box's own
+// far cells, shell 2 = its parent's far cells, and so on up to level 1. For a
+// box at level l these shells partition exactly the same set of bodies as the
+// reference list. Quotas go from the closest shell outward, each shell taking
+// at most half of the remaining budget (the last takes the rest); quota a
+// shell cannot use is then handed back from the closest shell outward. A
+// shell of n bodies (in cell CSR order, tree order within a cell) with quota
+// q keeps positions floor(t n / q), t = 0..q-1. The sample is emitted
+// farthest shell first. sampling <= 0, or a list no longer than s, keeps
+// every body in reference order.
 #pragma once
 
 #include <vector>
@@ -29,13 +36,18 @@ class FarBodies {
 
   FarBodies() = default;
   // level >= 1; boxes are the cells [2^level - 1, 2^(level+1) - 1).
-  FarBodies(const std::vector<Cell>& cells, const CSR& far, long long level, long long sampling);
+  // basis_far: the lists the reference gathers from; near/far: the matrix's
+  // theta lists that define the shells; with_near: basis_far covers every
+  // body outside the box (shell 0 present).
+  FarBodies(const std::vector<Cell>& cells, const CSR& basis_far, const CSR& near, const CSR& far, bool with_near,
+            long long level, long long sampling);
 
   long long fbodies_size_at_i(long long i) const { return offsets[i + 1] - offsets[i]; }
   const long long* fbodies_at_i(long long i) const { return indices.data() + offsets[i]; }
 };
 
 // Sampled far-body list of one cell (exposed for tests).
-std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR& far, long long cell, long long sampling);
+std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR& basis_far, const CSR& near,
+                                       const CSR& far, bool with_near, long long cell, long long sampling);
 
 }  // namespace mxc

@@ -47,7 +47,7 @@ void H2Matrix::construct(mxg_h2_t dev, const BasisOptions& opt, long long maxran
   LowerZ = 0;
 
   FarBodies fb;
-  if (lvl > 0) fb = FarBodies(cells, BasisFar, lvl, opt.sampling);
+  if (lvl > 0) fb = FarBodies(cells, BasisFar, Near, Far, opt.hss_far, lvl, opt.sampling);
   check(mxg_h2_construct_level(dev, lvl, nboxes, Dims.data(), lower ? nullptr : leaf_begin.data(), ARows.data(),
                                ACols.data(), CRows.data(), CCols.data(), lvl > 0 ? fb.offsets.data() : nullptr,
                                lvl > 0 ? fb.indices.data() : nullptr, opt.tol, maxrank, opt.orth ? 1 : 0, DimsLr.data()));

@@ -27,16 +27,22 @@ void mesh_sphere(double* bodies, long long nbodies, double r) {
 }
 
 void sphere_surface_rnd(double* bodies, long long nbodies, unsigned int seed) {
+  // golden-angle points on the unit sphere, then a uniform random rotation:
+  // unit quaternion from three U(0, 1) draws (Shoemake 1992)
+  mesh_sphere(bodies, nbodies, 1.);
   std::mt19937 gen(seed);
   std::uniform_real_distribution<double> dist(0., 1.);
   const double two_pi = 6.283185307179586476925286766559;
-  for (long long b = 0; b < nbodies; b++) {
-    double z = 2. * dist(gen) - 1.;
-    double phi = two_pi * dist(gen);
-    double r = std::sqrt(std::max(0., 1. - z * z));
-    bodies[3 * b] = r * std::cos(phi);
-    bodies[3 * b + 1] = r * std::sin(phi);
-    bodies[3 * b + 2] = z;
+  const double u1 = dist(gen), u2 = dist(gen), u3 = dist(gen);
+  const double a = std::sqrt(1. - u1), b = std::sqrt(u1);
+  const double qx = a * std::sin(two_pi * u2), qy = a * std::cos(two_pi * u2);
+  const double qz = b * std::sin(two_pi * u3), qw = b * std::cos(two_pi * u3);
+  const double R[3][3] = {{1. - 2. * (qy * qy + qz * qz), 2. * (qx * qy - qz * qw), 2. * (qx * qz + qy * qw)},
+                          {2. * (qx * qy + qz * qw), 1. - 2. * (qx * qx + qz * qz), 2. * (qy * qz - qx * qw)},
+                          {2. * (qx * qz - qy * qw), 2. * (qy * qz + qx * qw), 1. - 2. * (qx * qx + qy * qy)}};
+  for (long long i = 0; i < nbodies; i++) {
+    const double p0 = bodies[3 * i], p1 = bodies[3 * i + 1], p2 = bodies[3 * i + 2];
+    for (int d = 0; d < 3; d++) bodies[3 * i + d] = (R[d][0] * p0 + R[d][1] * p1) + R[d][2] * p2;
   }
 }
 

@@ -13,8 +13,13 @@ void uniform_unit_cube_rnd(double* bodies, long long nbodies, double diameter, l
 // test_funcs.hpp:43-61: golden-angle points on a sphere of radius r.
 void mesh_sphere(double* bodies, long long nbodies, double r);
 
-// Points uniform on the unit sphere (no reference generator exists):
-// z = 2u - 1, phi = 2 pi v with u then v from std::mt19937(seed) through U(0, 1).
+// Points uniform on the unit sphere (the reference has no such generator):
+// the golden-angle points of mesh_sphere at radius 1 under a uniform random
+// rotation, the unit quaternion drawn from three U(0, 1) values of
+// std::mt19937(seed) (Shoemake 1992). Quasi-uniform points keep every pair
+// at least ~sqrt(4 pi / N) apart; i.i.d. uniform points produce
+// near-coincident pairs whose 1/r entries swamp a fixed diagonal and make
+// the kernel matrix indefinite.
 void sphere_surface_rnd(double* bodies, long long nbodies, unsigned int seed);
 
 // Entries U(lo, hi) from std::mt19937_64(seed) (main.cpp:73-76 uses the default seed, U(0, 1)).

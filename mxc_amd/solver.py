@@ -58,12 +58,14 @@ def csr(far: bool, theta: float, levels: int, child, body, R, Cc):
     return rp, col[:nnz]
 
 
-def far_sample(levels: int, body, far_rp, far_col, cell: int, sampling: int) -> np.ndarray:
+def far_sample(levels: int, body, bfar, near, far, with_near: bool, cell: int, sampling: int) -> np.ndarray:
+    """Sampled far-body list of one cell; bfar/near/far are (rowptr, col) CSR pairs."""
     L = N.h2()
-    n = L.mxc_far_sample(levels, body.ctypes.data, far_rp.ctypes.data, far_col.ctypes.data, cell, sampling, None, 0)
+    args = (levels, body.ctypes.data, bfar[0].ctypes.data, bfar[1].ctypes.data, near[0].ctypes.data,
+            near[1].ctypes.data, far[0].ctypes.data, far[1].ctypes.data, int(with_near), cell, sampling)
+    n = L.mxc_far_sample(*args, None, 0)
     out = np.zeros(max(n, 1), np.int64)
-    L.mxc_far_sample(levels, body.ctypes.data, far_rp.ctypes.data, far_col.ctypes.data, cell, sampling,
-                     out.ctypes.data, n)
+    L.mxc_far_sample(*args, out.ctypes.data, n)
     return out[:n]
 
 

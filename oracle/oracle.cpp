@@ -137,53 +137,63 @@ long lookup(const std::vector<long>& rp, const std::vector<long>& col, long i, l
 }
 
 // Far-body list of cell `cell` (h-matrix.cpp:121-127,160-165: the parent's list
-// followed by the bodies of the cell's own Far-row cells, in CSR order), as
+// followed by the bodies of the cell's own basis-far cells, in CSR order), as
 // global body indices, reduced to at most `sampling` entries.
-// Sampling rule (builder-defined): the list is a sequence of per-level segments
-// (coarsest ancestor first). Quotas are assigned from the box's own level
-// upwards, each level taking at most half of the remaining budget (the
-// coarsest takes the rest); unused quota is then handed back from the finest
-// level up. Within a segment of n bodies with quota q, positions floor(t*n/q),
-// t = 0..q-1, are kept. sampling <= 0 or a list no longer than sampling keeps
-// the full list in reference order.
-std::vector<long> far_sample(const std::vector<Cell>& cells, const CSR& far, long cell, long sampling) {
+// Sampling rule (builder-defined): when the list is longer than `sampling`,
+// its bodies are grouped into distance shells from the matrix's theta lists:
+// shell 0 = the cell's near cells except itself (only when `with_near`, i.e.
+// the basis list holds every body outside the cell), then the far cells of
+// the cell, of its parent, ..., of its level-1 ancestor. Quotas run from the
+// closest shell outward, each taking at most half of the remaining budget
+// (the last shell takes the rest); unused quota is handed back from the
+// closest shell outward. A shell of n bodies with quota q keeps positions
+// floor(t*n/q), t = 0..q-1; shells are emitted farthest first. sampling <= 0
+// or a list no longer than sampling keeps the full list in reference order.
+std::vector<long> far_sample(const std::vector<Cell>& cells, const CSR& bfar, const CSR& near, const CSR& far, bool with_near,
+                             long cell, long sampling) {
   std::vector<long> chain;
   for (long c = cell; c > 0; c = (c - 1) / 2) chain.push_back(c);
   std::reverse(chain.begin(), chain.end());  // level 1 .. own level
-  long nseg = chain.size();
-  std::vector<long> segn(nseg, 0);
   long total = 0;
-  for (long a = 0; a < nseg; a++) {
-    long c = chain[a];
-    for (long k = far.rp[c]; k < far.rp[c + 1]; k++) {
-      const Cell& f = cells[far.col[k]];
-      segn[a] += f.body[1] - f.body[0];
-    }
-    total += segn[a];
-  }
-  std::vector<long> q(segn);
-  if (sampling > 0 && total > sampling) {
-    long rem = sampling;
-    for (long a = nseg - 1; a >= 0; a--) {
-      long share = (a == 0) ? rem : (rem + 1) / 2;
-      q[a] = std::min(segn[a], share);
-      rem -= q[a];
-    }
-    for (long a = nseg - 1; a >= 0 && rem > 0; a--) {
-      long extra = std::min(segn[a] - q[a], rem);
-      q[a] += extra;
-      rem -= extra;
-    }
-  }
+  for (long c : chain)
+    for (long k = bfar.rp[c]; k < bfar.rp[c + 1]; k++) total += cells[bfar.col[k]].body[1] - cells[bfar.col[k]].body[0];
   std::vector<long> out;
-  for (long a = 0; a < nseg; a++) {
+  if (sampling <= 0 || total <= sampling) {
+    for (long c : chain)
+      for (long k = bfar.rp[c]; k < bfar.rp[c + 1]; k++)
+        for (long b = cells[bfar.col[k]].body[0]; b < cells[bfar.col[k]].body[1]; b++) out.push_back(b);
+    return out;
+  }
+  // shells, closest first: lists of far cells
+  std::vector<std::vector<long>> shells;
+  if (with_near) {
+    shells.emplace_back();
+    for (long k = near.rp[cell]; k < near.rp[cell + 1]; k++)
+      if (near.col[k] != cell) shells.back().push_back(near.col[k]);
+  }
+  for (long a = (long)chain.size() - 1; a >= 0; a--) {
+    shells.emplace_back();
+    for (long k = far.rp[chain[a]]; k < far.rp[chain[a] + 1]; k++) shells.back().push_back(far.col[k]);
+  }
+  long nsh = shells.size();
+  std::vector<long> segn(nsh, 0), q(nsh, 0);
+  for (long a = 0; a < nsh; a++)
+    for (long f : shells[a]) segn[a] += cells[f].body[1] - cells[f].body[0];
+  long rem = sampling;
+  for (long a = 0; a < nsh; a++) {
+    long share = (a == nsh - 1) ? rem : (rem + 1) / 2;
+    q[a] = std::min(segn[a], share);
+    rem -= q[a];
+  }
+  for (long a = 0; a < nsh && rem > 0; a++) {
+    long extra = std::min(segn[a] - q[a], rem);
+    q[a] += extra;
+    rem -= extra;
+  }
+  for (long a = nsh - 1; a >= 0; a--) {
     if (q[a] == 0) continue;
-    long c = chain[a];
     std::vector<std::pair<long, long>> ranges;
-    for (long k = far.rp[c]; k < far.rp[c + 1]; k++) {
-      const Cell& f = cells[far.col[k]];
-      ranges.emplace_back(f.body[0], f.body[1]);
-    }
+    for (long f : shells[a]) ranges.emplace_back(cells[f].body[0], cells[f].body[1]);
     long n = segn[a], qa = q[a];
     long r = 0, base = 0;
     for (long t = 0; t < qa; t++) {
@@ -224,6 +234,7 @@ struct orc_h2 {
   double theta = 0., tol = 0.;
   long rank = 0, leveled_rank = 0, sampling = 0;
   bool orth = false;
+  bool hss = false;  // basis far lists are the sibling lists (every body outside the box)
   int nthreads = 1;
   CSR near, far, basisfar;
   std::vector<Level> lv;
@@ -402,7 +413,7 @@ void construct_level(orc_h2& h, long l) {
       long M = Lv.dims[i];
       std::vector<double> F;
       if (l > 0) {
-        std::vector<long> idx = far_sample(p.cells, h.basisfar, Lv.gbegin + i, h.sampling);
+        std::vector<long> idx = far_sample(p.cells, h.basisfar, h.near, h.far, h.hss, Lv.gbegin + i, h.sampling);
         F.resize(3 * idx.size());
         for (size_t t = 0; t < idx.size(); t++)
           for (int d = 0; d < 3; d++) F[3 * t + d] = p.bodies[3 * idx[t] + d];
@@ -453,8 +464,13 @@ void construct_level(orc_h2& h, long l) {
 }
 
 // h2matrix.cpp:251-271
+// Threads of the per-box loops of the solve passes (each box is independent,
+// so the result does not depend on the count).
+int g_threads = 1;
+
 void up_pass(Level& Lv, const double* xin) {
   std::copy(xin, xin + Lv.lenX, Lv.X.begin());
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads)
   for (long i = 0; i < Lv.nb; i++) {
     long M = Lv.dims[i], N = Lv.ranks[i];
     if (N > 0) orc::gemv('T', M, N, 1., Lv.Q.data() + Lv.offQ[i], M, Lv.X.data() + Lv.offX[i], 0., Lv.Z.data() + Lv.offZ[i]);
@@ -463,6 +479,7 @@ void up_pass(Level& Lv, const double* xin) {
 
 // h2matrix.cpp:273-301
 void down_pass(Level& Lv, double* yout) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads)
   for (long i = 0; i < Lv.nb; i++) {
     long M = Lv.dims[i], K = Lv.ranks[i];
     double* y = Lv.Y.data() + Lv.offX[i];
@@ -483,6 +500,7 @@ void down_pass(Level& Lv, double* yout) {
 // h2matrix.cpp:303-344
 void leaf_pass(Level& Lv, double* x) {
   std::copy(x, x + Lv.lenX, Lv.X.begin());
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads)
   for (long i = 0; i < Lv.nb; i++) {
     long M = Lv.dims[i], K = Lv.ranks[i];
     double* y = Lv.Y.data() + Lv.offX[i];
@@ -596,10 +614,12 @@ void copy_next(Level& up, const Level& Lv) {
 // h2matrix.cpp:448-493
 void forward(Level& Lv, const double* xin) {
   std::copy(xin, xin + Lv.lenX, Lv.Y.begin());
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads)
   for (long i = 0; i < Lv.nb; i++) {
     long M = Lv.dims[i];
     if (M > 0) orc::gemv('N', M, M, 1., Lv.R.data() + Lv.offQ[i], M, Lv.Y.data() + Lv.offX[i], 0., Lv.X.data() + Lv.offX[i]);
   }
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads)
   for (long i = 0; i < Lv.nb; i++) {
     long M = Lv.dims[i], Ms = Lv.ranks[i];
     if (Ms <= 0) continue;
@@ -615,6 +635,7 @@ void forward(Level& Lv, const double* xin) {
 
 // h2matrix.cpp:495-537
 void backward(Level& Lv, double* yout) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads)
   for (long i = 0; i < Lv.nb; i++) {
     long M = Lv.dims[i], Ms = Lv.ranks[i], Mr = M - Ms;
     double* x = Lv.X.data() + Lv.offX[i];
@@ -625,6 +646,7 @@ void backward(Level& Lv, double* yout) {
         if (Ns > 0) orc::gemv('N', Mr, Ns, -1., Lv.A.data() + Lv.offA[ij] + Ms, M, Lv.W.data() + Lv.offZ[j], 1., x + Ms);
       }
   }
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads)
   for (long i = 0; i < Lv.nb; i++) {
     long M = Lv.dims[i];
     if (M > 0) orc::gemv('N', M, M, 1., Lv.Q.data() + Lv.offQ[i], M, Lv.X.data() + Lv.offX[i], 0., Lv.Y.data() + Lv.offX[i]);
@@ -644,20 +666,25 @@ void orc_uniform_cube_rnd(double* bodies, long long n, double diameter, long lon
     for (int i = 0; i < 3; i++) bodies[3 * b + i] = i < dim ? u(gen) : 0.;
 }
 
-// Builder-defined (no reference generator): uniform on the unit sphere by
-// z = 2u - 1, phi = 2 pi v (Archimedes), u then v drawn per point from
-// std::mt19937(seed) through uniform_real_distribution(0, 1).
+// Builder-defined (no reference generator): uniform on the unit sphere as the
+// golden-angle points (test_funcs.hpp:43-61) at radius 1 under a uniform
+// random rotation; the unit quaternion comes from three U(0, 1) draws of
+// std::mt19937(seed) (Shoemake 1992: q = (sqrt(1-u1) sin 2pi u2,
+// sqrt(1-u1) cos 2pi u2, sqrt(u1) sin 2pi u3, sqrt(u1) cos 2pi u3)).
 void orc_sphere_rnd(double* bodies, long long n, unsigned int seed) {
+  orc_mesh_sphere(bodies, n, 1.);
   std::mt19937 gen(seed);
   std::uniform_real_distribution<double> u(0., 1.);
   const double two_pi = 6.283185307179586476925286766559;
-  for (long long b = 0; b < n; b++) {
-    double z = 2. * u(gen) - 1.;
-    double phi = two_pi * u(gen);
-    double r = std::sqrt(std::max(0., 1. - z * z));
-    bodies[3 * b] = r * std::cos(phi);
-    bodies[3 * b + 1] = r * std::sin(phi);
-    bodies[3 * b + 2] = z;
+  double u1 = u(gen), u2 = u(gen), u3 = u(gen);
+  double x = std::sqrt(1. - u1) * std::sin(two_pi * u2), y = std::sqrt(1. - u1) * std::cos(two_pi * u2);
+  double z = std::sqrt(u1) * std::sin(two_pi * u3), w = std::sqrt(u1) * std::cos(two_pi * u3);
+  double R[9] = {1. - 2. * (y * y + z * z), 2. * (x * y - z * w), 2. * (x * z + y * w),
+                 2. * (x * y + z * w), 1. - 2. * (x * x + z * z), 2. * (y * z - x * w),
+                 2. * (x * z - y * w), 2. * (y * z + x * w), 1. - 2. * (x * x + y * y)};
+  for (long long i = 0; i < n; i++) {
+    double p[3] = {bodies[3 * i], bodies[3 * i + 1], bodies[3 * i + 2]};
+    for (int d = 0; d < 3; d++) bodies[3 * i + d] = (R[3 * d] * p[0] + R[3 * d + 1] * p[1]) + R[3 * d + 2] * p[2];
   }
 }
 
@@ -733,14 +760,20 @@ long long orc_csr(int far, double theta, long long levels, const long long* chil
   return nnz;
 }
 
-long long orc_far_sample(long long levels, const long long* body, const long long* far_rp, const long long* far_col,
-                         long long cell, long long sampling, long long* out, long long cap) {
+long long orc_far_sample(long long levels, const long long* body, const long long* bfar_rp, const long long* bfar_col,
+                         const long long* near_rp, const long long* near_col, const long long* far_rp,
+                         const long long* far_col, int with_near, long long cell, long long sampling, long long* out,
+                         long long cap) {
   long nc = (2L << levels) - 1;
   std::vector<Cell> cells = cells_from(levels, nullptr, body, nullptr, nullptr);
-  CSR f;
-  f.rp.assign(far_rp, far_rp + nc + 1);
-  f.col.assign(far_col, far_col + far_rp[nc]);
-  std::vector<long> idx = far_sample(cells, f, cell, sampling);
+  auto mk = [&](const long long* rp, const long long* col) {
+    CSR c;
+    c.rp.assign(rp, rp + nc + 1);
+    c.col.assign(col, col + rp[nc]);
+    return c;
+  };
+  CSR bf = mk(bfar_rp, bfar_col), nr = mk(near_rp, near_col), fr = mk(far_rp, far_col);
+  std::vector<long> idx = far_sample(cells, bf, nr, fr, with_near != 0, cell, sampling);
   if (cap >= (long long)idx.size())
     for (size_t t = 0; t < idx.size(); t++) out[t] = idx[t];
   return idx.size();
@@ -823,6 +856,7 @@ orc_h2* orc_h2_create(const orc_problem* p, int kid, double diag, double alpha, 
   h->near = make_csr(false, p->cells, theta);
   h->far = make_csr(true, p->cells, theta);
   h->basisfar = hss_far ? make_csr(true, p->cells, 0.) : h->far;
+  h->hss = hss_far != 0;
   h->lv.resize(h->L + 1);
   for (long l = h->L; l >= 0; l--) construct_level(*h, l);
   return h;
@@ -843,6 +877,7 @@ long long orc_h2_far_count(const orc_h2* h, long long level) { return h->lv[leve
 // solver.cpp:76-88
 void orc_h2_matvec(orc_h2* h, double* x) {
   long L = h->L;
+  g_threads = h->nthreads;
   up_pass(h->lv[L], x);
   for (long l = L - 1; l >= 0; l--) up_pass(h->lv[l], h->lv[l + 1].Z.data());
   for (long l = 0; l < L; l++) down_pass(h->lv[l], h->lv[l + 1].W.data());
@@ -863,6 +898,7 @@ int orc_h2_factorize(orc_h2* h) {
 // solver.cpp:121-132
 void orc_h2_precondition(orc_h2* h, double* x) {
   long L = h->L;
+  g_threads = h->nthreads;
   forward(h->lv[L], x);
   for (long l = L - 1; l >= 0; l--) forward(h->lv[l], h->lv[l + 1].Z.data());
   for (long l = 0; l < L; l++) backward(h->lv[l], h->lv[l + 1].W.data());

@@ -29,8 +29,10 @@ void orc_build_tree(double* bodies, long long n, long long levels, long long* ch
                     double* R, double* C);
 long long orc_csr(int far, double theta, long long levels, const long long* child, const double* R,
                   const double* C, long long* rowptr, long long* col, long long cap);
-long long orc_far_sample(long long levels, const long long* body, const long long* far_rp, const long long* far_col,
-                         long long cell, long long sampling, long long* out, long long cap);
+long long orc_far_sample(long long levels, const long long* body, const long long* bfar_rp, const long long* bfar_col,
+                         const long long* near_rp, const long long* near_col, const long long* far_rp,
+                         const long long* far_col, int with_near, long long cell, long long sampling, long long* out,
+                         long long cap);
 
 /* kernel entries */
 double orc_kernel_eval(int kid, double diag, double alpha, double d);

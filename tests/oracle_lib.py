@@ -37,7 +37,7 @@ def lib() -> C.CDLL:
             "orc_levels": (c_ll, [c_ll, c_ll]),
             "orc_build_tree": (None, [P, c_ll, c_ll, P, P, P, P]),
             "orc_csr": (c_ll, [c_int, c_d, c_ll, P, P, P, P, P, c_ll]),
-            "orc_far_sample": (c_ll, [c_ll, P, P, P, c_ll, c_ll, P, c_ll]),
+            "orc_far_sample": (c_ll, [c_ll, P, P, P, P, P, P, P, c_int, c_ll, c_ll, P, c_ll]),
             "orc_kernel_eval": (c_d, [c_int, c_d, c_d, c_d]),
             "orc_gen_matrix": (None, [c_int, c_d, c_d, c_ll, c_ll, P, P, P]),
             "orc_dense_matvec": (None, [c_int, c_d, c_d, P, c_ll, P, P, c_int]),

@@ -65,11 +65,24 @@ def test_sphere_points_match_oracle_and_lie_on_sphere():
     n = 4000
     p = S.generate_points("sphere", n, 2)
     assert np.array_equal(p, O.sphere_points(n, 2))
-    r = np.linalg.norm(p.reshape(-1, 3), axis=1)
-    assert np.max(np.abs(r - 1)) < 1e-15
-    # uniform on the sphere: z ~ U(-1, 1), mean of z and of x^2 near 0 and 1/3
-    z = p.reshape(-1, 3)[:, 2]
-    assert abs(z.mean()) < 0.05 and abs((z**2).mean() - 1 / 3) < 0.02
+    P = p.reshape(-1, 3)
+    assert np.max(np.abs(np.linalg.norm(P, axis=1) - 1)) < 4e-16 * 8
+    # a rigid rotation of the golden-angle points: pairwise distances preserved
+    m = np.zeros(3 * n)
+    O.lib().orc_mesh_sphere(O.ptr(m), n, 1.0)
+    Mp = m.reshape(-1, 3)
+    idx = np.random.default_rng(0).choice(n, 200, replace=False)
+    dp = np.linalg.norm(P[idx, None] - P[None, idx], axis=-1)
+    dm = np.linalg.norm(Mp[idx, None] - Mp[None, idx], axis=-1)
+    assert np.max(np.abs(dp - dm)) < 1e-14
+    # the rotation (fit by least squares) is proper orthogonal and depends on the seed
+    Rm, *_ = np.linalg.lstsq(Mp, P, rcond=None)
+    assert np.allclose(Rm.T @ Rm, np.eye(3), atol=1e-12) and abs(np.linalg.det(Rm) - 1) < 1e-12
+    assert not np.allclose(S.generate_points("sphere", n, 3), p)
+    # uniform on the sphere: each coordinate has mean 0 and second moment 1/3
+    assert np.all(np.abs(P.mean(0)) < 0.01) and np.all(np.abs((P**2).mean(0) - 1 / 3) < 0.01)
+    # quasi-uniform: no pair closer than half the mean spacing
+    assert np.min(dm + 9 * np.eye(200)) > 0.5 * np.sqrt(4 * np.pi / n)
 
 
 def test_mesh_sphere_and_rhs_match_oracle():
@@ -189,32 +202,51 @@ def full_far_list(levels, body, frp, fcol, cell):
     return out
 
 
-@pytest.mark.parametrize("theta", [0.0, 2.0])
+def cell_bodies(body, cells):
+    out = set()
+    for f in cells:
+        out |= set(range(body[2 * f], body[2 * f + 1]))
+    return out
+
+
+@pytest.mark.parametrize("hss", [False, True])
 @pytest.mark.parametrize("sampling", [0, 100, 2000, 5000])
-def test_far_sample_matches_oracle(theta, sampling):
-    n, leaf = 8192, 128
+def test_far_sample_matches_oracle(hss, sampling):
+    n, leaf, theta = 8192, 128, 2.0
     pts = S.generate_points("cube", n, 1)
     levels = S.tree_levels(n, leaf)
     bodies, child, body, R, Cc = O.build_tree(pts, levels)
-    frp, fcol = O.csr(True, theta, levels, child, R, Cc)
+    near = O.csr(False, theta, levels, child, R, Cc)
+    far = O.csr(True, theta, levels, child, R, Cc)
+    bfar = O.csr(True, 0.0, levels, child, R, Cc) if hss else far
     nc = (2 << levels) - 1
     rng = np.random.default_rng(0)
     for cell in list(rng.choice(nc, 24, replace=False)) + [0, 1, 2, nc - 1]:
         cell = int(cell)
-        mine = S.far_sample(levels, body, frp, fcol, cell, sampling)
+        mine = S.far_sample(levels, body, bfar, near, far, hss, cell, sampling)
         out = np.zeros(n, np.int64)
-        k = O.lib().orc_far_sample(levels, O.ptr(body), O.ptr(frp), O.ptr(fcol), cell, sampling, O.ptr(out), n)
+        k = O.lib().orc_far_sample(levels, O.ptr(body), O.ptr(bfar[0]), O.ptr(bfar[1]), O.ptr(near[0]), O.ptr(near[1]),
+                                   O.ptr(far[0]), O.ptr(far[1]), int(hss), cell, sampling, O.ptr(out), n)
         assert np.array_equal(mine, out[:k])
-        full = full_far_list(levels, body, frp, fcol, cell)
+        full = full_far_list(levels, body, bfar[0], bfar[1], cell)
+        if hss and cell > 0:
+            lo, hi = body[2 * cell], body[2 * cell + 1]
+            assert sorted(full) == [b for b in range(n) if not lo <= b < hi]  # every body outside the cell
         if sampling <= 0 or len(full) <= sampling:
             assert list(mine) == full
-        else:
-            assert len(mine) == sampling
-            assert set(mine.tolist()) <= set(full)
-            assert len(set(mine.tolist())) == sampling  # distinct
-            # the own-level segment keeps the largest share
-            own = set()
-            for kk in range(frp[cell], frp[cell + 1]):
-                own |= set(range(body[2 * fcol[kk]], body[2 * fcol[kk] + 1]))
-            k_own = len(own & set(mine.tolist()))
-            assert k_own >= min(len(own), (sampling + 1) // 2)
+            continue
+        assert len(mine) == sampling
+        assert set(mine.tolist()) <= set(full)
+        assert len(set(mine.tolist())) == sampling  # distinct
+        # distance shells partition the reference list
+        shells = []
+        if hss:
+            shells.append(cell_bodies(body, [j for j in near[1][near[0][cell]:near[0][cell + 1]] if j != cell]))
+        a = cell
+        while a > 0:
+            shells.append(cell_bodies(body, far[1][far[0][a]:far[0][a + 1]]))
+            a = (a - 1) // 2
+        assert sum(len(s) for s in shells) == len(full) and set().union(*shells) == set(full)
+        # the closest shell keeps at least min(its size, half the budget)
+        got = set(mine.tolist())
+        assert len(shells[0] & got) >= min(len(shells[0]), (sampling + 1) // 2)
