@@ -6,6 +6,9 @@ namespace mxc {
 
 namespace {
 
+// closest distance shells kept whole in a far list
+constexpr long long kResolvedShells = 3;
+
 struct Shell {
   std::vector<std::pair<long long, long long>> ranges;  // body ranges of the shell's cells
   long long count = 0;
@@ -58,19 +61,26 @@ std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR
     for (long long k = far.RowIndex[path[a]]; k < far.RowIndex[path[a] + 1]; k++) sh.back().add(cells[far.ColIndex[k]]);
   }
 
-  long long left = sampling;
-  for (size_t k = 0; k < sh.size(); k++) {
-    long long cap = (k + 1 == sh.size()) ? left : (left + 1) / 2;
-    sh[k].quota = std::min(sh[k].count, cap);
-    left -= sh[k].quota;
-  }
-  for (size_t k = 0; k < sh.size() && left > 0; k++) {
-    long long more = std::min(sh[k].count - sh[k].quota, left);
-    sh[k].quota += more;
-    left -= more;
+  if (with_near) {
+    long long left = sampling;
+    for (size_t k = 0; k < sh.size(); k++) {
+      long long cap = (k + 1 == sh.size()) ? left : (left + 1) / 2;
+      sh[k].quota = std::min(sh[k].count, cap);
+      left -= sh[k].quota;
+    }
+    for (size_t k = 0; k < sh.size() && left > 0; k++) {
+      long long more = std::min(sh[k].count - sh[k].quota, left);
+      sh[k].quota += more;
+      left -= more;
+    }
+  } else {
+    for (size_t k = 0; k < sh.size(); k++)
+      sh[k].quota = (k < (size_t)kResolvedShells) ? sh[k].count : std::min(sh[k].count, sampling);
   }
 
-  out.reserve(sampling);
+  long long kept = 0;
+  for (const Shell& s : sh) kept += s.quota;
+  out.reserve(kept);
   for (size_t k = sh.size(); k-- > 0;) {
     const Shell& s = sh[k];
     size_t r = 0;

@@ -14,11 +14,17 @@
 // body outside the box, as with the sibling lists), shell 1 = the box's own
 // far cells, shell 2 = its parent's far cells, and so on up to level 1. For a
 // box at level l these shells partition exactly the same set of bodies as the
-// reference list. Quotas go from the closest shell outward, each shell taking
-// at most half of the remaining budget (the last takes the rest); quota a
-// shell cannot use is then handed back from the closest shell outward. A
-// shell of n bodies (in cell CSR order, tree order within a cell) with quota
-// q keeps positions floor(t n / q), t = 0..q-1. The sample is emitted
+// reference list. Quotas depend on what the list is for:
+//  * far lists of the accurate operator (no shell 0): the three closest
+//    shells are kept whole and every farther shell of n bodies keeps
+//    min(n, s) of them. The closest far cells carry the slowly decaying part
+//    of the far field, which an accurate (tol ~1e-12) basis must see in full;
+//  * lists of every outside body (the rank-capped preconditioner): s bodies
+//    in total, quotas going from the closest shell outward, each shell taking
+//    at most half of the remaining budget (the last takes the rest); quota a
+//    shell cannot use is then handed back from the closest shell outward.
+// A shell of n bodies (in cell CSR order, tree order within a cell) with
+// quota q keeps positions floor(t n / q), t = 0..q-1. The sample is emitted
 // farthest shell first. sampling <= 0, or a list no longer than s, keeps
 // every body in reference order.
 #pragma once

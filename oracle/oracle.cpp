@@ -19,6 +19,7 @@
 #include <random>
 #include <utility>
 #include <omp.h>
+#include <cstdlib>
 
 using orc::gemm;
 
@@ -138,17 +139,20 @@ long lookup(const std::vector<long>& rp, const std::vector<long>& col, long i, l
 
 // Far-body list of cell `cell` (h-matrix.cpp:121-127,160-165: the parent's list
 // followed by the bodies of the cell's own basis-far cells, in CSR order), as
-// global body indices, reduced to at most `sampling` entries.
+// global body indices, with the `sampling` cap.
 // Sampling rule (builder-defined): when the list is longer than `sampling`,
 // its bodies are grouped into distance shells from the matrix's theta lists:
 // shell 0 = the cell's near cells except itself (only when `with_near`, i.e.
 // the basis list holds every body outside the cell), then the far cells of
-// the cell, of its parent, ..., of its level-1 ancestor. Quotas run from the
-// closest shell outward, each taking at most half of the remaining budget
-// (the last shell takes the rest); unused quota is handed back from the
-// closest shell outward. A shell of n bodies with quota q keeps positions
-// floor(t*n/q), t = 0..q-1; shells are emitted farthest first. sampling <= 0
-// or a list no longer than sampling keeps the full list in reference order.
+// the cell, of its parent, ..., of its level-1 ancestor.
+//  * far lists (the accurate operator): the RESOLVED_SHELLS closest shells
+//    are kept whole, every farther shell keeps min(n, sampling) bodies;
+//  * lists of every outside body (the rank-capped preconditioner): a total
+//    of `sampling` bodies, quotas halving from the closest shell outward.
+// A shell of n bodies with quota q keeps positions floor(t*n/q),
+// t = 0..q-1; shells are emitted farthest first. sampling <= 0 or a list no
+// longer than sampling keeps the full list in reference order.
+const long RESOLVED_SHELLS = 3;
 std::vector<long> far_sample(const std::vector<Cell>& cells, const CSR& bfar, const CSR& near, const CSR& far, bool with_near,
                              long cell, long sampling) {
   std::vector<long> chain;
@@ -179,16 +183,22 @@ std::vector<long> far_sample(const std::vector<Cell>& cells, const CSR& bfar, co
   std::vector<long> segn(nsh, 0), q(nsh, 0);
   for (long a = 0; a < nsh; a++)
     for (long f : shells[a]) segn[a] += cells[f].body[1] - cells[f].body[0];
-  long rem = sampling;
-  for (long a = 0; a < nsh; a++) {
-    long share = (a == nsh - 1) ? rem : (rem + 1) / 2;
-    q[a] = std::min(segn[a], share);
-    rem -= q[a];
-  }
-  for (long a = 0; a < nsh && rem > 0; a++) {
-    long extra = std::min(segn[a] - q[a], rem);
-    q[a] += extra;
-    rem -= extra;
+  if (with_near) {
+    // halving quotas from the closest shell outward, the last shell takes the
+    // rest; unused quota is handed back from the closest shell outward
+    long rem = sampling;
+    for (long a = 0; a < nsh; a++) {
+      long share = (a == nsh - 1) ? rem : (rem + 1) / 2;
+      q[a] = std::min(segn[a], share);
+      rem -= q[a];
+    }
+    for (long a = 0; a < nsh && rem > 0; a++) {
+      long extra = std::min(segn[a] - q[a], rem);
+      q[a] += extra;
+      rem -= extra;
+    }
+  } else {
+    for (long a = 0; a < nsh; a++) q[a] = (a < RESOLVED_SHELLS) ? segn[a] : std::min(segn[a], sampling);
   }
   for (long a = nsh - 1; a >= 0; a--) {
     if (q[a] == 0) continue;
@@ -243,6 +253,23 @@ struct orc_h2 {
 
 namespace {
 
+// Row reduction of the far-field block (builder-defined): when a box has more
+// far rows N than columns M, the reference's R factor of the N x M block is
+// replaced by the M x M sketch Omega G with Rademacher entries
+// Omega(i, t) = sign of a splitmix64-style mix of (t, i). CPQR of R0 equals
+// CPQR of G in exact arithmetic; the sketch is a subspace embedding of G's
+// columns, so CPQR of Omega G is a randomized interpolative decomposition of
+// G (Halko, Martinsson, Tropp 2011). g_sketch = false restores the QR.
+bool g_sketch = true;
+
+inline double rademacher(unsigned long long t, unsigned long long i) {
+  unsigned long long z = t * 0x9E3779B97F4A7C15ull + i * 0xD1B54A32D192ED03ull + 0x632BE59BD9B4E019ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (z >> 63) ? -1. : 1.;
+}
+
 // h2matrix.cpp:13-63 compute_basis, generalized to (tol, maxrank, orth):
 //   reference tolerance mode: tol = eps, maxrank = 0, orth = false;
 //   reference fixed-rank mode: tol = 0, maxrank = rank_func(l), orth = true.
@@ -251,7 +278,19 @@ long compute_basis(const Kern& k, double tol, long maxrank, bool orth, long M, l
   long K = std::min(M, N), rank = 0;
   if (K <= 0) return 0;
   std::vector<double> RX((size_t)K * M, 0.);
-  if (K < N) {
+  if (K < N && g_sketch) {
+    // Y = Omega K(F, X) (M x M): row reduction by a Rademacher sketch, in
+    // chunks of far rows (replaces the R factor of h2matrix.cpp:78-79)
+    const long CH = 2048;
+    std::vector<double> XF((size_t)CH * M), Om((size_t)M * CH);
+    for (long t0 = 0; t0 < N; t0 += CH) {
+      long c = std::min(CH, N - t0);
+      gen_block(k, c, M, F + 3 * t0, X, XF.data(), c);
+      for (long t = 0; t < c; t++)
+        for (long i = 0; i < M; i++) Om[i + t * M] = rademacher(t0 + t, i);
+      gemm('N', 'N', M, M, c, 1., Om.data(), M, XF.data(), c, t0 == 0 ? 0. : 1., RX.data(), K);
+    }
+  } else if (K < N) {
     std::vector<double> XF((size_t)N * M);
     gen_block(k, N, M, F, X, XF.data(), N);
     std::vector<double> tau(std::min(N, M));
@@ -780,6 +819,9 @@ long long orc_far_sample(long long levels, const long long* body, const long lon
 }
 
 double orc_kernel_eval(int kid, double diag, double alpha, double d) { return Kern{kid, diag, alpha}(d); }
+
+void orc_set_sketch(int on) { g_sketch = on != 0; }
+double orc_rademacher(unsigned long long t, unsigned long long i) { return rademacher(t, i); }
 
 void orc_gen_matrix(int kid, double diag, double alpha, long long m, long long n, const double* bi, const double* bj, double* A) {
   gen_block(Kern{kid, diag, alpha}, m, n, bi, bj, A, m);

@@ -34,6 +34,10 @@ long long orc_far_sample(long long levels, const long long* body, const long lon
                          const long long* far_col, int with_near, long long cell, long long sampling, long long* out,
                          long long cap);
 
+/* far-row reduction: 1 = Rademacher sketch (default), 0 = the reference's Householder R factor */
+void orc_set_sketch(int on);
+double orc_rademacher(unsigned long long t, unsigned long long i);
+
 /* kernel entries */
 double orc_kernel_eval(int kid, double diag, double alpha, double d);
 void orc_gen_matrix(int kid, double diag, double alpha, long long m, long long n, const double* bi,

@@ -39,6 +39,8 @@ def lib() -> C.CDLL:
             "orc_csr": (c_ll, [c_int, c_d, c_ll, P, P, P, P, P, c_ll]),
             "orc_far_sample": (c_ll, [c_ll, P, P, P, P, P, P, P, c_int, c_ll, c_ll, P, c_ll]),
             "orc_kernel_eval": (c_d, [c_int, c_d, c_d, c_d]),
+            "orc_set_sketch": (None, [c_int]),
+            "orc_rademacher": (c_d, [C.c_ulonglong, C.c_ulonglong]),
             "orc_gen_matrix": (None, [c_int, c_d, c_d, c_ll, c_ll, P, P, P]),
             "orc_dense_matvec": (None, [c_int, c_d, c_d, P, c_ll, P, P, c_int]),
             "orc_qr_r": (None, [c_ll, c_ll, P, c_ll]),
