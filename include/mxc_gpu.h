@@ -79,7 +79,12 @@ int mxg_dense_matvec(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long
 /* ---- H^2 matrix: replaces createSpMatrixDesc/createDeviceCsr/createMatrixDesc and
  *      the numeric work of H2Matrix::construct (h2matrix.cpp:72-249). ---- */
 enum {
-  MXG_H2_FACTORABLE = 1 /* keep the assembled upper-level near blocks for mxg_ulv_create */
+  MXG_H2_FACTORABLE = 1, /* keep the assembled upper-level near blocks for mxg_ulv_create */
+  /* reduce a box's far rows (nf > M) by the Householder R factor of the far
+   * block, as the reference does (h2matrix.cpp:78-79), instead of the default
+   * M x M Rademacher sketch Omega * G (Omega(i, t) = -1 when bit i mod 64 of
+   * splitmix64-style word (t, i / 64) is set) */
+  MXG_H2_HOUSEHOLDER_ROWS = 2
 };
 /* bodies: host array, tree order (3 doubles per body). */
 int mxg_h2_create(mxg_ctx_t ctx, mxg_kernel_t k, const double* bodies, long long nbodies, long long levels,

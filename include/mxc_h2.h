@@ -66,6 +66,7 @@ typedef struct {
   int hss_far;  /* basis far lists from the sibling lists */
   long long sampling;  /* far bodies per box, <= 0 for all */
   int factorable;      /* keep what mxc_solver_factorize needs */
+  int householder_rows; /* reduce far rows by the Householder R factor (MXG_H2_HOUSEHOLDER_ROWS) */
 } mxc_h2_opts_t;
 typedef struct mxc_solver_s* mxc_solver_t;
 int mxc_solver_create(mxc_problem_t p, const mxc_h2_opts_t* o, mxc_solver_t* s);

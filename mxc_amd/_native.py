@@ -45,6 +45,7 @@ class MxcH2Opts(C.Structure):
         ("hss_far", c_int),
         ("sampling", c_ll),
         ("factorable", c_int),
+        ("householder_rows", c_int),
     ]
 
 

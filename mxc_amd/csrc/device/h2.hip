@@ -265,23 +265,35 @@ void construct_basis(mxg_h2_s* h, Level& Lv, long long l, const long long* far_o
   d_rank.alloc_zero(nb, s);
   std::fill(Lv.ranks.begin(), Lv.ranks.end(), 0);
 
+  // far rows are reduced by the Rademacher sketch unless the Householder R
+  // factor of the reference is requested
+  const bool sketch = !(h->flags & MXG_H2_HOUSEHOLDER_ROWS);
+  std::vector<long long> ldg(nb, 0);
+  for (long long i = 0; i < nb; i++) {
+    const long long M = Lv.dims[i];
+    ldg[i] = (sketch && nf[i] > M) ? M : nf[i];
+  }
+
   long long b0 = 0;
   while (b0 < nb) {
-    // chunk [b0, b1) of boxes whose G matrices fit the budget
+    // chunk [b0, b1) of boxes whose row-reduction buffers fit the budget
     long long b1 = b0;
-    size_t gsz = 0;
+    size_t gsz = 0, psz = 0;
     std::vector<size_t> goff;
     while (b1 < nb) {
-      long long M = Lv.dims[b1];
-      long long K = std::min(M, nf[b1]);
-      size_t need = K > 0 ? (size_t)nf[b1] * M : 0;
-      if (b1 > b0 && gsz + need > G_BUDGET) break;
+      const long long M = Lv.dims[b1];
+      const long long K = std::min(M, nf[b1]);
+      const size_t need = K > 0 ? (size_t)ldg[b1] * M : 0;
+      const size_t pneed = (K > 0 && sketch && nf[b1] > M) ? sketch_partial_size(M, nf[b1]) : 0;
+      if (b1 > b0 && gsz + need + psz + pneed > G_BUDGET) break;
       goff.push_back(gsz);
       gsz += need;
+      psz += pneed;
       b1++;
     }
-    DevBuf<double> G, tau, work;
+    DevBuf<double> G, tau, work, part;
     G.alloc(std::max<size_t>(gsz, 1));
+    part.alloc(std::max<size_t>(psz, 1));
     size_t tsz = 0, wsz = 0;
     for (long long i = b0; i < b1; i++) {
       tsz += (size_t)std::min(Lv.dims[i], nf[i]);
@@ -291,6 +303,7 @@ void construct_basis(mxg_h2_s* h, Level& Lv, long long l, const long long* far_o
     work.alloc(std::max<size_t>(wsz, 1));
 
     std::vector<GenTask> gens;
+    std::vector<SketchBox> sks;
     std::vector<QrTask> qrs;
     std::vector<CpqrTask> cps;
     size_t toff = 0, woff = 0;
@@ -298,15 +311,20 @@ void construct_basis(mxg_h2_s* h, Level& Lv, long long l, const long long* far_o
       const long long M = Lv.dims[i], K = std::min(M, nf[i]);
       if (K <= 0) continue;
       double* Gi = G.p + goff[i - b0];
-      gens.push_back(GenTask{d_far.p + far_off[i], Lv.S.p + Lv.offX[i], 0, 0, Gi, (int)nf[i], (int)M, (int)nf[i], 0});
-      if (K < nf[i]) qrs.push_back(QrTask{Gi, tau.p + toff, (int)nf[i], (int)M, (int)nf[i], 1});
+      if (sketch && nf[i] > M) {
+        sks.push_back(SketchBox{d_far.p + far_off[i], Lv.S.p + Lv.offX[i], Gi, nf[i], (int)M, 0});
+      } else {
+        gens.push_back(GenTask{d_far.p + far_off[i], Lv.S.p + Lv.offX[i], 0, 0, Gi, (int)nf[i], (int)M, (int)nf[i], 0});
+        if (K < nf[i]) qrs.push_back(QrTask{Gi, tau.p + toff, (int)nf[i], (int)M, (int)nf[i], 1});
+      }
       toff += K;
       long long steps = maxrank > 0 ? std::min(K, maxrank) : K;
-      cps.push_back(CpqrTask{Gi, d_perm.p + Lv.offX[i], d_rank.p + i, work.p + woff, (int)K, (int)M, (int)nf[i], (int)steps,
+      cps.push_back(CpqrTask{Gi, d_perm.p + Lv.offX[i], d_rank.p + i, work.p + woff, (int)K, (int)M, (int)ldg[i], (int)steps,
                              tol, (int)maxrank, 0});
       woff += 3 * M;
     }
     gen_tasks<double>(gens, h->bodies.p, h->k, s);
+    sketch_rows(sks, h->bodies.p, h->k, part.p, s);
     qr_batched(qrs, s);
     cpqr_batched(cps, s);
     std::vector<int> rk(nb);
@@ -343,7 +361,7 @@ void construct_basis(mxg_h2_s* h, Level& Lv, long long l, const long long* far_o
       pt.r = (int)r;
       if (0 < r && r < M) {
         pt.R0 = G.p + goff[i - b0];
-        pt.ldr = (int)nf[i];
+        pt.ldr = (int)ldg[i];
         pt.perm = d_perm.p + Lv.offX[i];
         pt.S = Lv.S.p + Lv.offX[i];
         pt.Stmp = d_stmp.p + Lv.offX[i];

@@ -94,6 +94,39 @@ struct LuTask {
 template <class T>
 void lu_inverse_batched(const std::vector<LuTask<T>>& tasks, hipStream_t s);
 
+// ---- Rademacher sketch of far-field blocks (sketch.hip) ----
+// Y (M x M, ld M) = Omega G, G(t, j) = K(|body(rows[t]) - body(cols[j])|),
+// t < nf, Omega(i, t) = -1 when bit (i mod 64) of the word mixed from
+// (t, i / 64) is set, else +1. Boxes with more than SKETCH_ROWS far rows are
+// summed from slices of SKETCH_ROWS rows held in a partial buffer of
+// sketch_partial_size(M, nf) doubles per box (in box order).
+constexpr int SKETCH_ROWS = 8192;
+struct SketchBox {
+  const int* rows;
+  const int* cols;
+  double* Y;
+  long long nf;
+  int M;
+  int pad;
+};
+struct SketchTask {
+  const int* rows;
+  const int* cols;
+  double* out;
+  int t0, t1, M, i0, j0;
+  int pad;
+};
+struct SliceSum {
+  const double* src;
+  double* dst;
+  long long len;
+  int nsplit;
+  int pad;
+};
+void sketch_rows(const std::vector<SketchBox>& boxes, const double* d_bodies, mxg_kernel_t k, double* d_partial,
+                 hipStream_t s);
+size_t sketch_partial_size(long long M, long long nf);
+
 // ---- misc kernels (h2.hip) ----
 template <class Tdst, class Tsrc>
 void convert(Tdst* dst, const Tsrc* src, size_t n, hipStream_t s);

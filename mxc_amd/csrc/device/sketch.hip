@@ -1,0 +1,181 @@
+// Row reduction of the far-field block by a Rademacher sketch.
+//
+// For a box with M columns (its bodies or skeleton) and nf > M far rows, the
+// shared-basis construction needs an M x M matrix with the row space of the
+// far block G(t, j) = K(|far_t - col_j|). The reference takes the R factor of
+// a Householder QR of G (h2matrix.cpp:78-79); here it is the sketch
+// Y = Omega G with Omega(i, t) = +-1, which has the same column relations in
+// exact arithmetic up to the subspace embedding (Halko, Martinsson, Tropp
+// 2011) and costs one streaming pass: G is never stored. Both operands are
+// generated in registers/LDS: the signs from a 64-bit mix of (t, i / 64), the
+// kernel entries from the body coordinates.
+//
+// Work decomposition: one 256-thread workgroup owns a 256 (sketch rows) x 64
+// (columns) tile of Y and a contiguous slice of far rows; slices of one box
+// are written as separate partial tiles and summed in a fixed order, so the
+// result is deterministic. Per step of 16 far rows the workgroup evaluates a
+// 16 x 64 block of G into LDS (4 entries per thread), computes the 16 sign
+// words its four waves need, and each wave runs 4 x 4 f64 MFMA 16x16x4 tiles
+// over its 64 sketch rows (v_mfma_f64_16x16x4f64, 64 accumulators per lane).
+#include "internal.hpp"
+
+namespace mxg {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int SK_TI = 256, SK_TJ = 64, SK_BK = 16;
+
+__device__ inline unsigned long long sketch_word(unsigned long long t, unsigned long long b) {
+  unsigned long long z = t * 0x9E3779B97F4A7C15ull + b * 0xD1B54A32D192ED03ull + 0x632BE59BD9B4E019ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ tasks, const double* __restrict__ bodies,
+                                                int kid, double diag, double alpha) {
+  __shared__ double Gs[SK_BK][SK_TJ + 1];
+  __shared__ unsigned long long Ws[SK_BK][4];
+  __shared__ double Xs[SK_TJ][3];
+  const SketchTask tk = tasks[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  if (tid < SK_TJ) {
+    const int j = tk.j0 + tid;
+    double x0 = 0., x1 = 0., x2 = 0.;
+    if (j < tk.M) {
+      const long long c = tk.cols[j];
+      x0 = bodies[3 * c];
+      x1 = bodies[3 * c + 1];
+      x2 = bodies[3 * c + 2];
+    }
+    Xs[tid][0] = x0;
+    Xs[tid][1] = x1;
+    Xs[tid][2] = x2;
+  }
+
+  d4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = d4{0., 0., 0., 0.};
+
+  const int irow0 = tk.i0 + wave * 64;
+  const bool active = irow0 < tk.M;
+  const int jj = tid & 63;
+  const bool jvalid = tk.j0 + jj < tk.M;
+
+  for (int tb = tk.t0; tb < tk.t1; tb += SK_BK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int tt = (tid >> 6) + 4 * q;
+      const int t = tb + tt;
+      double v = 0.;
+      if (t < tk.t1 && jvalid) {
+        const long long r = tk.rows[t];
+        const double p[3] = {bodies[3 * r], bodies[3 * r + 1], bodies[3 * r + 2]};
+        v = kernel_eval(kid, diag, alpha, dist2(p, Xs[jj]));
+      }
+      Gs[tt][jj] = v;
+    }
+    if (tid < 4 * SK_BK) {
+      const int tt = tid >> 2, b = tid & 3;
+      Ws[tt][b] = sketch_word((unsigned long long)(tb + tt), (unsigned long long)((tk.i0 >> 6) + b));
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int kk = 0; kk < SK_BK; kk += 4) {
+        const unsigned long long w = Ws[kk + lk][wave];
+        double a[4], g[4];
+#pragma unroll
+        for (int bi = 0; bi < 4; bi++) a[bi] = ((w >> (bi * 16 + lr)) & 1ull) ? -1. : 1.;
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++) g[bj] = Gs[kk + lk][bj * 16 + lr];
+#pragma unroll
+        for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+          for (int bj = 0; bj < 4; bj++)
+            acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], g[bj], acc[bi][bj], 0, 0, 0);
+      }
+    }
+  }
+
+  if (!active) return;
+#pragma unroll
+  for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int row = irow0 + bi * 16 + lk + 4 * q;
+        const int col = tk.j0 + bj * 16 + lr;
+        if (row < tk.M && col < tk.M) tk.out[row + (long long)col * tk.M] = acc[bi][bj][q];
+      }
+}
+
+// dst[e] = sum_{s < nsplit} src[e + s * len], in order of s.
+__global__ void __launch_bounds__(256) k_sum_slices(const SliceSum* __restrict__ tasks, const long long* __restrict__ prefix,
+                                                    int ntasks) {
+  const long long blk = blockIdx.x;
+  const int t = find_task(prefix, ntasks, blk);
+  const SliceSum tk = tasks[t];
+  const long long e = (blk - prefix[t]) * 256 + threadIdx.x;
+  if (e >= tk.len) return;
+  double s = 0.;
+  for (int k = 0; k < tk.nsplit; k++) s += tk.src[e + (long long)k * tk.len];
+  tk.dst[e] = s;
+}
+
+}  // namespace
+
+void sketch_rows(const std::vector<SketchBox>& boxes, const double* d_bodies, mxg_kernel_t k, double* d_partial,
+                 hipStream_t s) {
+  std::vector<SketchTask> tasks;
+  std::vector<SliceSum> sums;
+  std::vector<long long> prefix{0};
+  size_t poff = 0;
+  for (const SketchBox& b : boxes) {
+    if (b.M <= 0 || b.nf <= 0) continue;
+    const int nsplit = (int)((b.nf + SKETCH_ROWS - 1) / SKETCH_ROWS);
+    const long long len = (long long)b.M * b.M;
+    double* base = nsplit == 1 ? b.Y : d_partial + poff;
+    for (int sp = 0; sp < nsplit; sp++) {
+      const int t0 = sp * SKETCH_ROWS, t1 = (int)std::min<long long>(b.nf, (long long)(sp + 1) * SKETCH_ROWS);
+      for (int i0 = 0; i0 < b.M; i0 += SK_TI)
+        for (int j0 = 0; j0 < b.M; j0 += SK_TJ)
+          tasks.push_back(SketchTask{b.rows, b.cols, base + sp * len, t0, t1, b.M, i0, j0});
+    }
+    if (nsplit > 1) {
+      sums.push_back(SliceSum{base, b.Y, len, nsplit});
+      prefix.push_back(prefix.back() + (len + 255) / 256);
+      poff += (size_t)nsplit * len;
+    }
+  }
+  if (tasks.empty()) return;
+  MXG_REQUIRE(tasks.size() < (1u << 31), "sketch grid too large");
+  DevBuf<SketchTask> dt;
+  dt.upload(tasks, s);
+  hipLaunchKernelGGL(k_sketch, dim3((unsigned)tasks.size()), dim3(256), 0, s, dt.p, d_bodies, (int)k.id, k.diag, k.alpha);
+  MXG_HIP(hipGetLastError());
+  DevBuf<SliceSum> ds;
+  DevBuf<long long> dp;
+  if (!sums.empty()) {
+    ds.upload(sums, s);
+    dp.upload(prefix, s);
+    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)prefix.back()), dim3(256), 0, s, ds.p, dp.p, (int)sums.size());
+    MXG_HIP(hipGetLastError());
+  }
+  MXG_HIP(hipStreamSynchronize(s));
+}
+
+size_t sketch_partial_size(long long M, long long nf) {
+  if (M <= 0 || nf <= 0) return 0;
+  const long long nsplit = (nf + SKETCH_ROWS - 1) / SKETCH_ROWS;
+  return nsplit > 1 ? (size_t)nsplit * M * M : 0;
+}
+
+}  // namespace mxg

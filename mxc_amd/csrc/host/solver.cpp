@@ -39,7 +39,8 @@ void H2MatrixSolver::build(const MatrixAccessor& eval, const std::vector<Cell>& 
   CSR Near('N', cells, cells, theta);
   CSR Far('F', cells, cells, theta);
   CSR BasisFar = opts.hss_far ? CSR('F', cells, cells, 0.) : Far;
-  check(mxg_h2_create(ctx, eval.desc(), bodies, nbodies, levels, factorable ? MXG_H2_FACTORABLE : 0, &dev));
+  const int flags = (factorable ? MXG_H2_FACTORABLE : 0) | (opts.householder_rows ? MXG_H2_HOUSEHOLDER_ROWS : 0);
+  check(mxg_h2_create(ctx, eval.desc(), bodies, nbodies, levels, flags, &dev));
   A.assign(levels + 1, H2Matrix());
   for (long long l = levels; l >= 0; l--) {
     // rank_func(l) = (levels - l) * leveled_rank + rank (solver.cpp:29)

@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -19,6 +20,7 @@ class DeviceArray:
         p = C.c_void_p()
         N.check_gpu(N.gpu().mxg_malloc(ctx.handle, self.nbytes, C.byref(p)))
         self.ptr = p.value
+        ctx.adopt(self, Context.ARRAYS)
 
     @property
     def size(self) -> int:
@@ -41,9 +43,11 @@ class DeviceArray:
         return self
 
     def free(self) -> None:
-        if self.ptr:
+        if self.ptr and self.ctx.handle:
             N.gpu().mxg_free(self.ctx.handle, self.ptr)
-            self.ptr = None
+        self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:
@@ -53,12 +57,22 @@ class DeviceArray:
 
 
 class Context:
-    """One GPU: device selection and the stream every mxg call runs on."""
+    """One GPU: device selection and the stream every mxg call runs on.
+
+    Objects built on the context (solvers, problems, device arrays) register
+    here; close() releases the ones still alive, solvers first, before the
+    context itself, so no native handle outlives the context it was made on."""
+
+    SOLVERS, PROBLEMS, ARRAYS = 0, 1, 2
 
     def __init__(self, device: int = 0):
         h = C.c_void_p()
         N.check_gpu(N.gpu().mxg_init(device, C.byref(h)))
         self.handle = h.value
+        self._owned = [weakref.WeakSet() for _ in range(3)]
+
+    def adopt(self, obj, kind: int) -> None:
+        self._owned[kind].add(obj)
 
     def array(self, a: np.ndarray, dtype=None) -> DeviceArray:
         a = np.asarray(a)
@@ -76,6 +90,9 @@ class Context:
 
     def close(self) -> None:
         if self.handle:
+            for group in self._owned:
+                for obj in list(group):
+                    obj.close()
             N.gpu().mxg_finalize(self.handle)
             self.handle = None
 

@@ -82,42 +82,50 @@ class Problem:
         self.levels = int(N.h2().mxc_problem_levels(self.handle))
         self.bodies = np.zeros(3 * self.n)
         N.check_h2(N.h2().mxc_problem_bodies(self.handle, self.bodies.ctypes.data))
+        ctx.adopt(self, Context.PROBLEMS)
 
     def dense_matvec(self, kernel: str, diag: float, alpha: float, x: DeviceArray, y: DeviceArray) -> None:
         N.check_h2(N.h2().mxc_dense_matvec(self.handle, kernel_desc(kernel, diag, alpha), x.ptr, y.ptr))
 
     def close(self):
-        if self.handle:
+        if self.handle and self.ctx.handle:
             N.h2().mxc_problem_destroy(self.handle)
-            self.handle = None
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class H2:
     """An H^2 matrix on the device (H2MatrixSolver)."""
 
     def __init__(self, problem: Problem, kernel="laplace", diag=1e3, alpha=1.0, theta=2.0, tol=0.0, rank=0,
-                 leveled_rank=0, orth=False, hss_far=False, sampling=0, factorable=False):
+                 leveled_rank=0, orth=False, hss_far=False, sampling=0, factorable=False, householder_rows=False):
         self.problem = problem
         self.kernel = (kernel, diag, alpha)
         o = N.MxcH2Opts(N.KERNELS[kernel], diag, alpha, theta, tol, rank, leveled_rank, int(orth), int(hss_far), sampling,
-                        int(factorable))
+                        int(factorable), int(householder_rows))
         h = C.c_void_p()
         N.check_h2(N.h2().mxc_solver_create(problem.handle, C.byref(o), C.byref(h)))
         self.handle = h.value
         self.levels = problem.levels
+        problem.ctx.adopt(self, Context.SOLVERS)
 
     @classmethod
-    def operator(cls, problem, eps, sampling, kernel="laplace", diag=1e3, alpha=1.0, theta=2.0):
+    def operator(cls, problem, eps, sampling, kernel="laplace", diag=1e3, alpha=1.0, theta=2.0, householder_rows=False):
         """The accurate operator of lorasp: interpolative bases at tolerance eps (solver.cpp:28-36, epi < 1)."""
-        return cls(problem, kernel, diag, alpha, theta, tol=eps, sampling=sampling)
+        return cls(problem, kernel, diag, alpha, theta, tol=eps, sampling=sampling, householder_rows=householder_rows)
 
     @classmethod
     def preconditioner(cls, problem, rank, sampling, kernel="laplace", diag=1e3, alpha=1.0, theta=2.0, eps=0.0,
-                       leveled_rank=0):
+                       leveled_rank=0, householder_rows=False):
         """The ULV preconditioner of lorasp: orthogonal bases of fixed rank (epi == 0), or truncated at eps
         with the rank as a cap, basis far lists from the sibling lists (solver.cpp:28-36)."""
         return cls(problem, kernel, diag, alpha, theta, tol=eps, rank=rank, leveled_rank=leveled_rank, orth=True,
-                   hss_far=True, sampling=sampling, factorable=True)
+                   hss_far=True, sampling=sampling, factorable=True, householder_rows=householder_rows)
 
     def dims(self, level: int):
         nb = 1 << level
@@ -130,6 +138,14 @@ class H2:
         c = np.zeros(2, np.int64)
         N.check_h2(N.h2().mxc_solver_counts(self.handle, level, c.ctypes.data))
         return int(c[0]), int(c[1])
+
+    def download(self, level: int, what: int) -> np.ndarray:
+        """Level data on the host: what 0 = near blocks, 1 = bases Q, 2 = R, 3 = couplings, 4 = skeleton indices."""
+        dev = N.h2().mxc_solver_device(self.handle)
+        n = N.gpu().mxg_h2_level_size(dev, level, what)
+        out = np.zeros(max(n, 0))
+        N.check_gpu(N.gpu().mxg_h2_download(dev, level, what, out.ctypes.data, n))
+        return out
 
     def matvec(self, x: DeviceArray, y: DeviceArray, precision: int = N.MXG_FP64) -> None:
         N.check_h2(N.h2().mxc_solver_matvec(self.handle, x.ptr, y.ptr, precision))
@@ -156,9 +172,9 @@ class H2:
         return resid[: it.value + 1], it.value, inner.value
 
     def close(self):
-        if self.handle:
+        if self.handle and self.problem.ctx.handle:
             N.h2().mxc_solver_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

@@ -256,18 +256,23 @@ namespace {
 // Row reduction of the far-field block (builder-defined): when a box has more
 // far rows N than columns M, the reference's R factor of the N x M block is
 // replaced by the M x M sketch Omega G with Rademacher entries
-// Omega(i, t) = sign of a splitmix64-style mix of (t, i). CPQR of R0 equals
+// Omega(i, t) from a splitmix64-style mix of (t, i / 64). CPQR of R0 equals
 // CPQR of G in exact arithmetic; the sketch is a subspace embedding of G's
 // columns, so CPQR of Omega G is a randomized interpolative decomposition of
 // G (Halko, Martinsson, Tropp 2011). g_sketch = false restores the QR.
 bool g_sketch = true;
 
-inline double rademacher(unsigned long long t, unsigned long long i) {
-  unsigned long long z = t * 0x9E3779B97F4A7C15ull + i * 0xD1B54A32D192ED03ull + 0x632BE59BD9B4E019ull;
+// 64 sketch signs per word: Omega(i, t) = -1 when bit (i mod 64) of
+// sketch_word(t, i / 64) is set.
+inline unsigned long long sketch_word(unsigned long long t, unsigned long long b) {
+  unsigned long long z = t * 0x9E3779B97F4A7C15ull + b * 0xD1B54A32D192ED03ull + 0x632BE59BD9B4E019ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  z ^= z >> 31;
-  return (z >> 63) ? -1. : 1.;
+  return z ^ (z >> 31);
+}
+
+inline double rademacher(unsigned long long t, unsigned long long i) {
+  return ((sketch_word(t, i >> 6) >> (i & 63)) & 1) ? -1. : 1.;
 }
 
 // h2matrix.cpp:13-63 compute_basis, generalized to (tol, maxrank, orth):
@@ -912,6 +917,13 @@ void orc_h2_dims(const orc_h2* h, long long level, long long* dims, long long* r
     if (dims) dims[i] = Lv.dims[i];
     if (ranks) ranks[i] = Lv.ranks[i];
   }
+}
+// what: 1 = bases Q, 2 = R (M*M per box, column major, box order); returns the count
+long long orc_h2_download(const orc_h2* h, long long level, int what, double* out) {
+  const Level& Lv = h->lv[level];
+  const std::vector<double>& v = (what == 1) ? Lv.Q : Lv.R;
+  if (out) std::copy(v.begin(), v.end(), out);
+  return (long long)v.size();
 }
 long long orc_h2_near_count(const orc_h2* h, long long level) { return h->lv[level].arp.back(); }
 long long orc_h2_far_count(const orc_h2* h, long long level) { return h->lv[level].crp.back(); }

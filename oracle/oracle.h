@@ -68,6 +68,7 @@ orc_h2* orc_h2_create(const orc_problem* p, int kid, double diag, double alpha, 
 void orc_h2_destroy(orc_h2* h);
 long long orc_h2_nboxes(const orc_h2* h, long long level);
 void orc_h2_dims(const orc_h2* h, long long level, long long* dims, long long* ranks);
+long long orc_h2_download(const orc_h2* h, long long level, int what, double* out);
 long long orc_h2_near_count(const orc_h2* h, long long level);
 long long orc_h2_far_count(const orc_h2* h, long long level);
 void orc_h2_matvec(orc_h2* h, double* x);

@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
             "orc_h2_nboxes": (c_ll, [P, c_ll]),
             "orc_h2_dims": (None, [P, c_ll, P, P]),
             "orc_h2_near_count": (c_ll, [P, c_ll]),
+            "orc_h2_download": (c_ll, [P, c_ll, c_int, P]),
             "orc_h2_far_count": (c_ll, [P, c_ll]),
             "orc_h2_matvec": (None, [P, P]),
             "orc_h2_factorize": (c_int, [P]),
@@ -154,6 +155,11 @@ class H2:
         r = np.zeros(nb, np.int64)
         lib().orc_h2_dims(self.h, level, ptr(d), ptr(r))
         return d, r
+
+    def download(self, level: int, what: int) -> np.ndarray:
+        out = np.zeros(lib().orc_h2_download(self.h, level, what, None))
+        lib().orc_h2_download(self.h, level, what, ptr(out))
+        return out
 
     def matvec(self, x: np.ndarray) -> np.ndarray:
         y = np.ascontiguousarray(x, dtype=np.float64).copy()
