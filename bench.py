@@ -152,7 +152,7 @@ def main():
         "dtype": "f32 ULV + f32 inner operator, f64 Arnoldi/residual",
         "data": "synthetic (seeded points and right-hand sides)",
         "config": {
-            "workload": "H2 GMRES solve, sphere Yukawa N=2^20 (configs[2])",
+            "workload": "H2 GMRES solve, unit-sphere Yukawa N=2^20, leaf 256",
             "N": n, "leaf": leaf, "theta": theta, "kernel": "yukawa exp(-r)/r, K(0)=1e3",
             "operator_eps": args.op_eps, "sampling": args.sampling, "precond_eps": 1e-6, "precond_max_rank": 64,
             "restart": restart, "tol": tol, "levels": L,

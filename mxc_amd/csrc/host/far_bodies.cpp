@@ -6,8 +6,15 @@ namespace mxc {
 
 namespace {
 
-// closest distance shells kept whole in a far list
-constexpr long long kResolvedShells = 3;
+// Closest distance shells kept whole in a far list of a tree with `levels`
+// levels: half the depth, at least three.
+long long resolvedShells(long long levels) { return std::max(3ll, (levels + 1) / 2); }
+
+long long treeLevels(const std::vector<Cell>& cells) {
+  long long l = 0;
+  while ((2ll << l) - 1 < (long long)cells.size()) l++;
+  return l;
+}
 
 struct Shell {
   std::vector<std::pair<long long, long long>> ranges;  // body ranges of the shell's cells
@@ -74,8 +81,8 @@ std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR
       left -= more;
     }
   } else {
-    for (size_t k = 0; k < sh.size(); k++)
-      sh[k].quota = (k < (size_t)kResolvedShells) ? sh[k].count : std::min(sh[k].count, sampling);
+    const size_t resolved = (size_t)resolvedShells(treeLevels(cells));
+    for (size_t k = 0; k < sh.size(); k++) sh[k].quota = (k < resolved) ? sh[k].count : std::min(sh[k].count, sampling);
   }
 
   long long kept = 0;

@@ -15,10 +15,13 @@
 // far cells, shell 2 = its parent's far cells, and so on up to level 1. For a
 // box at level l these shells partition exactly the same set of bodies as the
 // reference list. Quotas depend on what the list is for:
-//  * far lists of the accurate operator (no shell 0): the three closest
-//    shells are kept whole and every farther shell of n bodies keeps
-//    min(n, s) of them. The closest far cells carry the slowly decaying part
-//    of the far field, which an accurate (tol ~1e-12) basis must see in full;
+//  * far lists of the accurate operator (no shell 0): the R closest shells
+//    are kept whole, R = max(3, ceil(L / 2)) for a tree of L levels, and
+//    every farther shell of n bodies keeps min(n, s) of them. A basis built
+//    to tol ~1e-12 must see the nearer far cells in full: with R = 3 the
+//    operator error of the unit-sphere Yukawa problem grows from 1.4e-13
+//    (L = 8) to 4.5e-10 (L = 10) and 1.1e-9 (L = 12); with R = ceil(L / 2)
+//    it is 1.7e-13 (L = 10) and 2.3e-13 (L = 12);
 //  * lists of every outside body (the rank-capped preconditioner): s bodies
 //    in total, quotas going from the closest shell outward, each shell taking
 //    at most half of the remaining budget (the last takes the rest); quota a

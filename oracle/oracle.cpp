@@ -145,14 +145,15 @@ long lookup(const std::vector<long>& rp, const std::vector<long>& col, long i, l
 // shell 0 = the cell's near cells except itself (only when `with_near`, i.e.
 // the basis list holds every body outside the cell), then the far cells of
 // the cell, of its parent, ..., of its level-1 ancestor.
-//  * far lists (the accurate operator): the RESOLVED_SHELLS closest shells
-//    are kept whole, every farther shell keeps min(n, sampling) bodies;
+//  * far lists (the accurate operator): the R = max(3, ceil(L / 2)) closest
+//    shells are kept whole (L = tree levels), every farther shell keeps
+//    min(n, sampling) bodies;
 //  * lists of every outside body (the rank-capped preconditioner): a total
 //    of `sampling` bodies, quotas halving from the closest shell outward.
 // A shell of n bodies with quota q keeps positions floor(t*n/q),
 // t = 0..q-1; shells are emitted farthest first. sampling <= 0 or a list no
 // longer than sampling keeps the full list in reference order.
-const long RESOLVED_SHELLS = 3;
+long resolved_shells(long levels) { return std::max(3L, (levels + 1) / 2); }
 std::vector<long> far_sample(const std::vector<Cell>& cells, const CSR& bfar, const CSR& near, const CSR& far, bool with_near,
                              long cell, long sampling) {
   std::vector<long> chain;
@@ -198,7 +199,10 @@ std::vector<long> far_sample(const std::vector<Cell>& cells, const CSR& bfar, co
       rem -= extra;
     }
   } else {
-    for (long a = 0; a < nsh; a++) q[a] = (a < RESOLVED_SHELLS) ? segn[a] : std::min(segn[a], sampling);
+    long levels = 0;
+    while ((2L << levels) - 1 < (long)cells.size()) levels++;
+    const long resolved = resolved_shells(levels);
+    for (long a = 0; a < nsh; a++) q[a] = (a < resolved) ? segn[a] : std::min(segn[a], sampling);
   }
   for (long a = nsh - 1; a >= 0; a--) {
     if (q[a] == 0) continue;
@@ -936,6 +940,60 @@ void orc_h2_matvec(orc_h2* h, double* x) {
   for (long l = L - 1; l >= 0; l--) up_pass(h->lv[l], h->lv[l + 1].Z.data());
   for (long l = 0; l < L; l++) down_pass(h->lv[l], h->lv[l + 1].W.data());
   leaf_pass(h->lv[L], x);
+}
+
+// Far field of one level alone: y_approx = the H^2 matvec restricted to the
+// couplings of `level` (upward pass, couplings of that level, downward pass
+// without couplings, no near field); y_exact = the same interactions summed
+// directly, sum over far pairs (i, j) of the level of K(X_i, X_j) x_j.
+// A diagnostic that attributes the construction error to a level.
+void orc_h2_far_level(orc_h2* h, long long level, const double* x, double* y_approx, double* y_exact) {
+  const long L = h->L;
+  g_threads = h->nthreads;
+  up_pass(h->lv[L], x);
+  for (long l = L - 1; l >= level; l--) up_pass(h->lv[l], h->lv[l + 1].Z.data());
+  Level& T = h->lv[level];
+  std::vector<double> y(T.lenX, 0.);
+  for (long i = 0; i < T.nb; i++) {
+    long M = T.dims[i], K = T.ranks[i];
+    if (K == 0) continue;
+    std::vector<double> w(K, 0.);
+    for (long ij = T.crp[i]; ij < T.crp[i + 1]; ij++) {
+      long j = T.ccol[ij];
+      orc::gemv('N', K, T.ranks[j], 1., T.C.data() + T.offC[ij], K, T.Z.data() + T.offZ[j], 1., w.data());
+    }
+    orc::gemv('N', M, K, 1., T.Q.data() + T.offQ[i], M, w.data(), 0., y.data() + T.offX[i]);
+  }
+  for (long l = level + 1; l <= L; l++) {
+    Level& D = h->lv[l];
+    std::vector<double> yl(D.lenX, 0.);
+    for (long i = 0; i < D.nb; i++) {
+      long M = D.dims[i], K = D.ranks[i];
+      if (K > 0) orc::gemv('N', M, K, 1., D.Q.data() + D.offQ[i], M, y.data() + D.offZ[i], 0., yl.data() + D.offX[i]);
+    }
+    y.swap(yl);
+  }
+  std::copy(y.begin(), y.end(), y_approx);
+  // direct sums
+  const std::vector<Cell>& cells = h->p->cells;
+  const double* B = h->p->bodies.data();
+  const long first = (1L << level) - 1;
+  std::fill(y_exact, y_exact + h->lv[L].lenX, 0.);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(g_threads)
+  for (long i = 0; i < T.nb; i++) {
+    const Cell& ci = cells[first + i];
+    for (long ij = T.crp[i]; ij < T.crp[i + 1]; ij++) {
+      const Cell& cj = cells[first + T.ccol[ij]];
+      for (long a = ci.body[0]; a < ci.body[1]; a++) {
+        double s = 0.;
+        for (long b = cj.body[0]; b < cj.body[1]; b++) {
+          double dx = B[3 * a] - B[3 * b], dy = B[3 * a + 1] - B[3 * b + 1], dz = B[3 * a + 2] - B[3 * b + 2];
+          s += h->k(std::sqrt(dx * dx + dy * dy + dz * dz)) * x[b];
+        }
+        y_exact[a] += s;
+      }
+    }
+  }
 }
 
 // solver.cpp:90-100

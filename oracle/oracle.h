@@ -73,6 +73,8 @@ long long orc_h2_near_count(const orc_h2* h, long long level);
 long long orc_h2_far_count(const orc_h2* h, long long level);
 void orc_h2_matvec(orc_h2* h, double* x);
 int orc_h2_factorize(orc_h2* h);
+/* diagnostic: the far field of one level, through the H^2 bases and directly */
+void orc_h2_far_level(orc_h2* h, long long level, const double* x, double* y_approx, double* y_exact);
 void orc_h2_precondition(orc_h2* h, double* x);
 long long orc_h2_gmres(orc_h2* A, orc_h2* M, double tol, double* x, const double* b, long long inner,
                        long long outer, double* resid);

@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
             "orc_h2_far_count": (c_ll, [P, c_ll]),
             "orc_h2_matvec": (None, [P, P]),
             "orc_h2_factorize": (c_int, [P]),
+            "orc_h2_far_level": (None, [P, c_ll, P, P, P]),
             "orc_h2_precondition": (None, [P, P]),
             "orc_h2_gmres": (c_ll, [P, P, c_d, P, P, c_ll, c_ll, P]),
             "orc_rel_err": (c_d, [c_ll, P, P]),
@@ -165,6 +166,13 @@ class H2:
         y = np.ascontiguousarray(x, dtype=np.float64).copy()
         lib().orc_h2_matvec(self.h, ptr(y))
         return y
+
+    def far_level(self, level: int, x: np.ndarray):
+        """(approximate, exact) far field of the couplings of one level."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        ya, ye = np.zeros_like(x), np.zeros_like(x)
+        lib().orc_h2_far_level(self.h, level, ptr(x), ptr(ya), ptr(ye))
+        return ya, ye
 
     def factorize(self) -> int:
         return lib().orc_h2_factorize(self.h)

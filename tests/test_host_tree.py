@@ -211,8 +211,9 @@ def cell_bodies(body, cells):
 
 @pytest.mark.parametrize("hss", [False, True])
 @pytest.mark.parametrize("sampling", [0, 100, 2000, 5000])
-def test_far_sample_matches_oracle(hss, sampling):
-    n, leaf, theta = 8192, 128, 2.0
+@pytest.mark.parametrize("n,leaf", [(8192, 128), (32768, 32)])
+def test_far_sample_matches_oracle(hss, sampling, n, leaf):
+    theta = 2.0
     pts = S.generate_points("cube", n, 1)
     levels = S.tree_levels(n, leaf)
     bodies, child, body, R, Cc = O.build_tree(pts, levels)
@@ -253,8 +254,10 @@ def test_far_sample_matches_oracle(hss, sampling):
             assert len(mine) == sampling
             assert len(shells[0] & got) >= min(len(shells[0]), (sampling + 1) // 2)
         else:
-            # three closest shells whole, every farther shell min(size, sampling)
+            # the max(3, ceil(levels / 2)) closest shells whole, every farther
+            # shell min(size, sampling)
+            res = max(3, (levels + 1) // 2)
             for k, sk in enumerate(shells):
-                want = len(sk) if k < 3 else min(len(sk), sampling)
+                want = len(sk) if k < res else min(len(sk), sampling)
                 assert len(sk & got) == want
-            assert len(mine) == sum(len(sk) if k < 3 else min(len(sk), sampling) for k, sk in enumerate(shells))
+            assert len(mine) == sum(len(sk) if k < res else min(len(sk), sampling) for k, sk in enumerate(shells))
