@@ -47,6 +47,11 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q))
 
 
+def log(msg):
+    """Progress on stderr; stdout carries only the JSON line."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -68,9 +73,11 @@ def main():
     prob = Problem(ctx, pts, leaf)
     A = H2.operator(prob, args.op_eps, args.sampling, kernel, diag, alpha, theta)
     t_op = time.time() - t0
+    log(f"operator built in {t_op:.1f} s")
     t0 = time.time()
     M = H2.preconditioner(prob, 64, args.sampling, kernel, diag, alpha, theta, eps=1e-6)
     t_pc = time.time() - t0
+    log(f"preconditioner built in {t_pc:.1f} s")
     L = prob.levels
 
     rhs = [ctx.array(uniform_vector(n, -1.0, 1.0, 1000 + k)) for k in range(args.warmup + args.steps)]
@@ -112,6 +119,7 @@ def main():
             N.check_gpu(N.gpu().mxg_profile_read(ctx.handle, cls, C.byref(cnt), C.byref(ms)))
             kern[name] = {"launches": cnt.value, "total_ms": ms.value,
                           "avg_ms": ms.value / cnt.value if cnt.value else None}
+        log(f"factor {factor_prec} inner {inner_prec}: {args.steps} solves in {elapsed:.2f} s")
         # dense FP64 check of every solution (outside the timed region)
         dense = []
         for k in range(args.steps):
@@ -119,6 +127,7 @@ def main():
             ax = work.download()
             b = rhs[args.warmup + k].download()
             dense.append(float(np.linalg.norm(b - ax) / np.linalg.norm(b)))
+            log(f"dense check {k}: {dense[-1]:.3e}")
         return dict(elapsed=elapsed, lat=lat, cycles=cycles, inner=inners, h2_resid=res, dense=dense, factor_s=fsec,
                     kernels=kern)
 
@@ -183,6 +192,7 @@ def main():
         },
     }
     if not args.no_cpu_baseline:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.sampling, args.op_eps, n)
     print(json.dumps(out))
 
