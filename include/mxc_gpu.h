@@ -154,11 +154,11 @@ int mxg_gmres(mxg_h2_t A, mxg_ulv_t M, const mxg_gmres_opts_t* opts, const doubl
  * by a pair of HIP events; mxg_profile_read synchronizes and returns the
  * number of launches and their summed device time since the last reset. */
 enum {
-  MXG_PROF_MATVEC_LEAF_FP64 = 0, /* leaf pass of the H^2 matvec: near field + couplings + basis */
+  MXG_PROF_MATVEC_LEAF_FP64 = 0, /* near-field product of the leaf level (one kernel launch) */
   MXG_PROF_MATVEC_LEAF_FP32 = 1,
   MXG_PROF_ULV_FWD_LEAF = 2,     /* forward substitution, leaf level (x = V y and z) */
   MXG_PROF_ULV_BWD_LEAF = 3,     /* backward substitution, leaf level */
-  MXG_PROF_MATVEC_UPPER = 4,     /* all other matvec passes */
+  MXG_PROF_MATVEC_UPPER = 4,     /* every other matvec launch: bases, couplings, per-box sums */
   MXG_PROF_CLASSES = 5
 };
 int mxg_profile_enable(mxg_ctx_t ctx, int enable);

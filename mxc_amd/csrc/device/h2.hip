@@ -117,67 +117,19 @@ __global__ void __launch_bounds__(256) k_post(const PostTask* __restrict__ tasks
   }
 }
 
-// ---------------- matvec kernels (one workgroup per box) ----------------
-
-template <class T>
-__global__ void __launch_bounds__(256) k_up(const int* __restrict__ dims, const int* __restrict__ ranks,
-                                            const long long* __restrict__ offQ, const long long* __restrict__ offX,
-                                            const long long* __restrict__ offZ, const T* __restrict__ Q,
-                                            const T* __restrict__ x, T* __restrict__ z) {
+// QT_i = Q_i[:, :k]^T (k x M, column major), the row-major basis the upward
+// pass streams with the block-sparse product kernel.
+__global__ void __launch_bounds__(256) k_basis_transpose(const int* __restrict__ dims, const int* __restrict__ ranks,
+                                                         const long long* __restrict__ offQ,
+                                                         const long long* __restrict__ offQT, const double* __restrict__ Q,
+                                                         double* __restrict__ QT) {
   const int i = blockIdx.x;
   const int M = dims[i], k = ranks[i];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const T* Qi = Q + offQ[i];
-  const T* xi = x + offX[i];
-  for (int c = wave; c < k; c += 4) {
-    const T* q = Qi + (long long)c * M;
-    T s = 0;
-    for (int r = lane; r < M; r += 64) s += q[r] * xi[r];
-    s = wave_sum(s);
-    if (lane == 0) z[offZ[i] + c] = s;
-  }
-}
-
-// w_i (+)= sum_far C_ij z_j into LDS, then y_i = Q_i[:, :k] w_i (+ near field).
-template <class T, bool LEAF>
-__global__ void __launch_bounds__(256) k_down(const int* __restrict__ dims, const int* __restrict__ ranks,
-                                              const long long* __restrict__ offQ, const long long* __restrict__ offX,
-                                              const long long* __restrict__ offZ, const long long* __restrict__ frp,
-                                              const int* __restrict__ fcol, const long long* __restrict__ offC,
-                                              const long long* __restrict__ nrp, const int* __restrict__ ncol,
-                                              const long long* __restrict__ offA, const T* __restrict__ Q,
-                                              const T* __restrict__ C, const T* __restrict__ A, const T* __restrict__ z,
-                                              const T* __restrict__ w_in, const T* __restrict__ x, T* __restrict__ y) {
-  extern __shared__ unsigned char smem[];
-  T* sw = reinterpret_cast<T*>(smem);
-  const int i = blockIdx.x;
-  const int M = dims[i], k = ranks[i];
-  for (int r = threadIdx.x; r < k; r += 256) {
-    T s = w_in[offZ[i] + r];
-    for (long long ij = frp[i]; ij < frp[i + 1]; ij++) {
-      const int j = fcol[ij];
-      const int kj = ranks[j];
-      const T* c = C + offC[ij] + r;
-      const T* zj = z + offZ[j];
-      for (int cc = 0; cc < kj; cc++) s += c[(long long)cc * k] * zj[cc];
-    }
-    sw[r] = s;
-  }
-  __syncthreads();
-  const T* Qi = Q + offQ[i];
-  for (int r = threadIdx.x; r < M; r += 256) {
-    T s = 0;
-    for (int c = 0; c < k; c++) s += Qi[r + (long long)c * M] * sw[c];
-    if (LEAF) {
-      for (long long ij = nrp[i]; ij < nrp[i + 1]; ij++) {
-        const int j = ncol[ij];
-        const int Mj = dims[j];
-        const T* a = A + offA[ij] + r;
-        const T* xj = x + offX[j];
-        for (int c = 0; c < Mj; c++) s += a[(long long)c * M] * xj[c];
-      }
-    }
-    y[offX[i] + r] = s;
+  const double* Qi = Q + offQ[i];
+  double* Ti = QT + offQT[i];
+  for (long long e = threadIdx.x; e < (long long)M * k; e += 256) {
+    const int r = (int)(e % M), c = (int)(e / M);
+    Ti[c + (long long)r * k] = Qi[e];
   }
 }
 
@@ -472,50 +424,107 @@ void upload_indices(mxg_h2_s* h, Level& Lv) {
 
 }  // namespace
 
+// Matvec task lists of a constructed level and the term offsets of its
+// downward pass; sizes the shared term scratch.
+void build_matvec_lists(mxg_h2_s* h, Level& Lv, bool leaf) {
+  hipStream_t s = h->ctx->stream;
+  const long long nb = Lv.nb;
+  Lv.offQT.assign(nb + 1, 0);
+  for (long long i = 0; i < nb; i++) Lv.offQT[i + 1] = Lv.offQT[i] + Lv.dims[i] * Lv.ranks[i];
+  Lv.QT.alloc(std::max<long long>(Lv.offQT[nb], 1));
+  DevBuf<long long> d_offQT;
+  d_offQT.upload(Lv.offQT, s);
+  hipLaunchKernelGGL(k_basis_transpose, dim3((unsigned)nb), dim3(256), 0, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p, d_offQT.p,
+                     Lv.Q.p, Lv.QT.p);
+  MXG_HIP(hipGetLastError());
+  std::vector<PairTask> up, far, near;
+  for (long long i = 0; i < nb; i++)
+    if (Lv.ranks[i] > 0)
+      up.push_back(PairTask{Lv.offQT[i], Lv.offX[i], Lv.offZ[i], (int)Lv.ranks[i], (int)Lv.dims[i], (int)Lv.ranks[i], 0});
+  Lv.toffF.assign(Lv.frp[nb] + 1, 0);
+  for (long long i = 0; i < nb; i++)
+    for (long long p = Lv.frp[i]; p < Lv.frp[i + 1]; p++) {
+      const long long j = Lv.fcol[p];
+      Lv.toffF[p + 1] = Lv.toffF[p] + Lv.ranks[i];
+      if (Lv.ranks[i] > 0)
+        far.push_back(PairTask{Lv.offC[p], Lv.offZ[j], Lv.toffF[p], (int)Lv.ranks[i], (int)Lv.ranks[j], (int)Lv.ranks[i], 0});
+    }
+  Lv.lenTF = Lv.toffF.back();
+  Lv.toffN.assign(Lv.nrp[nb] + 1, 0);
+  if (leaf) {
+    for (long long i = 0; i < nb; i++)
+      for (long long p = Lv.nrp[i]; p < Lv.nrp[i + 1]; p++) {
+        const long long j = Lv.ncol[p];
+        Lv.toffN[p + 1] = Lv.toffN[p] + Lv.dims[i];
+        if (Lv.dims[i] > 0)
+          near.push_back(PairTask{Lv.offA[p], Lv.offX[j], Lv.toffN[p], (int)Lv.dims[i], (int)Lv.dims[j], (int)Lv.dims[i], 0});
+      }
+    Lv.lenTN = Lv.toffN.back();
+  }
+  pair_list_build(Lv.up_pl, up, Lv.offQT[nb], Lv.lenX, Lv.lenZ, s);
+  pair_list_build(Lv.far_pl, far, (long long)Lv.C.n, Lv.lenZ, Lv.lenTF, s);
+  if (leaf) pair_list_build(Lv.near_pl, near, (long long)Lv.A.n, h->n, Lv.lenTN, s);
+  Lv.d_toffF.upload(Lv.toffF, s);
+  Lv.d_toffN.upload(Lv.toffN, s);
+  Lv.vecQ[0] = Lv.vecQ[1] = true;
+  for (long long i = 0; i < nb; i++) {
+    if (Lv.dims[i] % 2) Lv.vecQ[0] = false;
+    if (Lv.dims[i] % 4) Lv.vecQ[1] = false;
+  }
+  h->t64.reserve((size_t)std::max<long long>(Lv.lenTF + Lv.lenTN, 1));
+  MXG_HIP(hipStreamSynchronize(s));
+}
+
 template <class T>
 void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y) {
   hipStream_t s = h->ctx->stream;
   const long long L = h->L;
+  constexpr bool f64 = sizeof(T) == 8;
   auto Zp = [&](long long l) -> T* {
-    if constexpr (sizeof(T) == 8) return reinterpret_cast<T*>(h->Z[l].p); else return reinterpret_cast<T*>(h->Z32[l].p);
+    if constexpr (f64) return reinterpret_cast<T*>(h->Z[l].p); else return reinterpret_cast<T*>(h->Z32[l].p);
   };
   auto Wp = [&](long long l) -> T* {
-    if constexpr (sizeof(T) == 8) return reinterpret_cast<T*>(h->W[l].p); else return reinterpret_cast<T*>(h->W32[l].p);
+    if constexpr (f64) return reinterpret_cast<T*>(h->W[l].p); else return reinterpret_cast<T*>(h->W32[l].p);
   };
   auto Qp = [&](Level& Lv) -> const T* {
-    if constexpr (sizeof(T) == 8) return reinterpret_cast<const T*>(Lv.Q.p); else return reinterpret_cast<const T*>(Lv.Q32.p);
+    if constexpr (f64) return reinterpret_cast<const T*>(Lv.Q.p); else return reinterpret_cast<const T*>(Lv.Q32.p);
+  };
+  auto QTp = [&](Level& Lv) -> const T* {
+    if constexpr (f64) return reinterpret_cast<const T*>(Lv.QT.p); else return reinterpret_cast<const T*>(Lv.QT32.p);
   };
   auto Cp = [&](Level& Lv) -> const T* {
-    if constexpr (sizeof(T) == 8) return reinterpret_cast<const T*>(Lv.C.p); else return reinterpret_cast<const T*>(Lv.C32.p);
+    if constexpr (f64) return reinterpret_cast<const T*>(Lv.C.p); else return reinterpret_cast<const T*>(Lv.C32.p);
   };
   auto Ap = [&](Level& Lv) -> const T* {
-    if constexpr (sizeof(T) == 8) return reinterpret_cast<const T*>(Lv.A.p); else return reinterpret_cast<const T*>(Lv.A32.p);
+    if constexpr (f64) return reinterpret_cast<const T*>(Lv.A.p); else return reinterpret_cast<const T*>(Lv.A32.p);
   };
+  T* tbuf;
+  if constexpr (f64) tbuf = h->t64.p; else tbuf = h->t32.p;
+  // upward pass: z_i = Q_i[:, :k]^T x_i
   for (long long l = L; l >= 0; l--) {
     Level& Lv = h->lv[l];
-    const T* xin = (l == L) ? x : Zp(l + 1);
     if (Lv.lenZ == 0) continue;
+    const T* xin = (l == L) ? x : Zp(l + 1);
     ProfScope prof(h->ctx, MXG_PROF_MATVEC_UPPER);
-    hipLaunchKernelGGL(k_up<T>, dim3((unsigned)Lv.nb), dim3(256), 0, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p, Lv.d_offX.p,
-                       Lv.d_offZ.p, Qp(Lv), xin, Zp(l));
-    MXG_HIP(hipGetLastError());
+    pair_gemv<T>(Lv.up_pl, QTp(Lv), xin, Zp(l), s);
   }
+  // downward pass: couplings into w, then y = Q w (+ near field at the leaf)
   for (long long l = 0; l <= L; l++) {
     Level& Lv = h->lv[l];
     const bool leaf = (l == L);
     T* out = leaf ? y : Wp(l + 1);
-    size_t shm = std::max<long long>(Lv.maxK, 1) * sizeof(T);
-    ProfScope prof(h->ctx, leaf ? (sizeof(T) == 8 ? MXG_PROF_MATVEC_LEAF_FP64 : MXG_PROF_MATVEC_LEAF_FP32)
-                                : MXG_PROF_MATVEC_UPPER);
-    if (leaf)
-      hipLaunchKernelGGL((k_down<T, true>), dim3((unsigned)Lv.nb), dim3(256), shm, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p,
-                         Lv.d_offX.p, Lv.d_offZ.p, Lv.d_frp.p, Lv.d_fcol.p, Lv.d_offC.p, Lv.d_nrp.p, Lv.d_ncol.p, Lv.d_offA.p,
-                         Qp(Lv), Cp(Lv), Ap(Lv), Zp(l), Wp(l), x, out);
-    else
-      hipLaunchKernelGGL((k_down<T, false>), dim3((unsigned)Lv.nb), dim3(256), shm, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p,
-                         Lv.d_offX.p, Lv.d_offZ.p, Lv.d_frp.p, Lv.d_fcol.p, Lv.d_offC.p, Lv.d_nrp.p, Lv.d_ncol.p, Lv.d_offA.p,
-                         Qp(Lv), Cp(Lv), Ap(Lv), Zp(l), Wp(l), x, out);
-    MXG_HIP(hipGetLastError());
+    T* tF = tbuf;
+    T* tN = tbuf + Lv.lenTF;
+    {
+      ProfScope prof(h->ctx, MXG_PROF_MATVEC_UPPER);
+      pair_gemv<T>(Lv.far_pl, Cp(Lv), Zp(l), tF, s);
+    }
+    if (leaf) {
+      ProfScope prof(h->ctx, f64 ? MXG_PROF_MATVEC_LEAF_FP64 : MXG_PROF_MATVEC_LEAF_FP32);
+      pair_gemv<T>(Lv.near_pl, Ap(Lv), x, tN, s);
+    }
+    ProfScope prof(h->ctx, MXG_PROF_MATVEC_UPPER);
+    comb_down<T>(Lv, Lv.vecQ[f64 ? 0 : 1], Lv.d_toffF.p, Lv.d_toffN.p, Qp(Lv), Wp(l), tF, leaf ? tN : nullptr, out, s);
   }
 }
 
@@ -539,6 +548,8 @@ void make_shadow32(mxg_h2_s* h) {
     convert(Lv.Q32.p, Lv.Q.p, Lv.Q.n, s);
     Lv.C32.alloc(Lv.C.n);
     convert(Lv.C32.p, Lv.C.p, Lv.C.n, s);
+    Lv.QT32.alloc(Lv.QT.n);
+    convert(Lv.QT32.p, Lv.QT.p, Lv.QT.n, s);
     if (l == h->L) {
       Lv.A32.alloc(Lv.A.n);
       convert(Lv.A32.p, Lv.A.p, Lv.A.n, s);
@@ -548,6 +559,7 @@ void make_shadow32(mxg_h2_s* h) {
   }
   h->x32.alloc(h->n);
   h->y32.alloc(h->n);
+  h->t32.alloc(std::max<size_t>(h->t64.n, 1));
   MXG_HIP(hipStreamSynchronize(s));
   h->shadow32 = true;
 }
@@ -731,6 +743,7 @@ int mxg_h2_construct_level(mxg_h2_t h, long long l, long long nb, const long lon
     Lv.C.alloc_zero(Lv.offC[nnzC], s);
     construct_coupling(h, Lv, orth);
     upload_indices(h, Lv);
+    build_matvec_lists(h, Lv, !low);
     h->Z[l].alloc_zero(std::max<long long>(Lv.lenZ, 1), s);
     h->W[l].alloc_zero(std::max<long long>(Lv.lenZ, 1), s);
     MXG_HIP(hipStreamSynchronize(s));

@@ -17,6 +17,22 @@
 
 namespace mxg {
 
+// One dense product term t = B v of a block-sparse pass: B is an n x m
+// column-major (sub-)block with leading dimension ld; b, v and out are
+// element offsets from the base pointers given at launch.
+struct PairTask {
+  long long b, v, out;
+  int n, m, ld;
+  int pad;
+};
+// A task list resident on the device. vec[0] / vec[1]: every block admits
+// 16-byte column loads in FP64 / FP32 (rows, ld and offset multiples of 2 / 4).
+struct PairList {
+  DevBuf<PairTask> tasks;
+  long long count = 0;
+  bool vec[2] = {false, false};
+};
+
 struct Level {
   long long nb = 0;
   bool built = false;
@@ -33,6 +49,15 @@ struct Level {
   std::vector<int> up_ld;
   // FP32 copies of the operator data (inner matvec of mixed precision GMRES)
   DevBuf<float> A32, Q32, C32;
+  // matvec task lists: up z_i = QT_i x_i (QT_i = Q_i[:, :k]^T, k x M), far
+  // t_p = C_p z_j, leaf near u_p = A_p x_j; toffF / toffN: term offsets
+  std::vector<long long> offQT, toffF, toffN;
+  DevBuf<double> QT;
+  DevBuf<float> QT32;
+  PairList up_pl, far_pl, near_pl;
+  DevBuf<long long> d_toffF, d_toffN;
+  long long lenTF = 0, lenTN = 0;
+  bool vecQ[2] = {false, false};  // every M_i admits 16-byte loads of Q columns (FP64 / FP32)
 };
 
 }  // namespace mxg
@@ -48,6 +73,9 @@ struct mxg_h2_s {
   std::vector<mxg::DevBuf<float>> Z32, W32;
   mxg::DevBuf<float> x32, y32;
   bool shadow32 = false;
+  // scratch for the product terms of one level's passes
+  mxg::DevBuf<double> t64;
+  mxg::DevBuf<float> t32;
 };
 
 namespace mxg {
@@ -57,4 +85,26 @@ void copy2d_batched(const std::vector<const T*>& src, const std::vector<T*>& dst
                     const std::vector<int>& n, const std::vector<int>& lds, const std::vector<int>& ldd, hipStream_t s);
 template <class T>
 void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y);
+
+// ---- block-sparse products (bgemv.hip) ----
+template <class T>
+bool pair_tasks_vectorizable(const std::vector<PairTask>& tasks);
+// Uploads tasks into pl and sets its alignment flags, after checking every
+// task against the sizes (elements) of the matrix, input and output buffers.
+void pair_list_build(PairList& pl, const std::vector<PairTask>& tasks, long long nB, long long nv, long long nout,
+                     hipStream_t s);
+// out[t.out + r] = sum_c B[t.b + r + c*ld] v[t.v + c] for every task
+template <class T>
+void pair_gemv(const PairList& pl, const T* B, const T* v, T* out, hipStream_t s);
+// w_i = w_in_i + sum_far tF, y_i = Q_i[:, :k] w_i (+ sum_near tN when tN != nullptr)
+template <class T>
+void comb_down(const Level& Lv, bool vec, const long long* d_toffF, const long long* d_toffN, const T* Q, const T* w_in,
+               const T* tF, const T* tN, T* y, hipStream_t s);
+// z_i = x_i[:s] - sum_near t
+template <class T>
+void comb_fwd(const Level& Lv, const long long* d_toff, const T* x, const T* t, T* z, hipStream_t s);
+// y_i = U_i [w_i ; x_i[s:] - sum_near t]
+template <class T>
+void comb_bwd(const Level& Lv, bool vec, const long long* d_toff, const T* U, const T* x, const T* w, const T* t, T* y,
+              hipStream_t s);
 }  // namespace mxg

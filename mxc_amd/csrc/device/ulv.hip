@@ -44,78 +44,6 @@ __global__ void k_transpose_rows(const int* __restrict__ dims, const int* __rest
   }
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) k_fwd1(const int* __restrict__ dims, const long long* __restrict__ offQ,
-                                              const long long* __restrict__ offX, const T* __restrict__ V,
-                                              const T* __restrict__ y, T* __restrict__ x) {
-  const int i = blockIdx.x;
-  const int M = dims[i];
-  const T* Vi = V + offQ[i];
-  const T* yi = y + offX[i];
-  for (int r = threadIdx.x; r < M; r += 256) {
-    T s = 0;
-    for (int c = 0; c < M; c++) s += Vi[r + (long long)c * M] * yi[c];
-    x[offX[i] + r] = s;
-  }
-}
-
-template <class T>
-__global__ void __launch_bounds__(256) k_fwd2(const int* __restrict__ dims, const int* __restrict__ ranks,
-                                              const long long* __restrict__ offX, const long long* __restrict__ offZ,
-                                              const long long* __restrict__ nrp, const int* __restrict__ ncol,
-                                              const long long* __restrict__ offA, const T* __restrict__ A,
-                                              const T* __restrict__ x, T* __restrict__ z) {
-  const int i = blockIdx.x;
-  const int M = dims[i], s = ranks[i];
-  for (int r = threadIdx.x; r < s; r += 256) {
-    T acc = x[offX[i] + r];
-    for (long long ij = nrp[i]; ij < nrp[i + 1]; ij++) {
-      const int j = ncol[ij];
-      const int N = dims[j], sj = ranks[j];
-      const T* a = A + offA[ij] + r + (long long)sj * M;
-      const T* xj = x + offX[j] + sj;
-      for (int c = 0; c < N - sj; c++) acc -= a[(long long)c * M] * xj[c];
-    }
-    z[offZ[i] + r] = acc;
-  }
-}
-
-template <class T>
-__global__ void __launch_bounds__(256) k_bwd(const int* __restrict__ dims, const int* __restrict__ ranks,
-                                             const long long* __restrict__ offQ, const long long* __restrict__ offX,
-                                             const long long* __restrict__ offZ, const long long* __restrict__ nrp,
-                                             const int* __restrict__ ncol, const long long* __restrict__ offA,
-                                             const T* __restrict__ A, const T* __restrict__ U, const T* __restrict__ x,
-                                             const T* __restrict__ w, T* __restrict__ y) {
-  extern __shared__ unsigned char smem[];
-  T* sx = reinterpret_cast<T*>(smem);
-  const int i = blockIdx.x;
-  const int M = dims[i], s = ranks[i];
-  for (int r = threadIdx.x; r < M; r += 256) {
-    T acc;
-    if (r < s) {
-      acc = w[offZ[i] + r];
-    } else {
-      acc = x[offX[i] + r];
-      for (long long ij = nrp[i]; ij < nrp[i + 1]; ij++) {
-        const int j = ncol[ij];
-        const int sj = ranks[j];
-        const T* a = A + offA[ij] + r;
-        const T* wj = w + offZ[j];
-        for (int c = 0; c < sj; c++) acc -= a[(long long)c * M] * wj[c];
-      }
-    }
-    sx[r] = acc;
-  }
-  __syncthreads();
-  const T* Ui = U + offQ[i];
-  for (int r = threadIdx.x; r < M; r += 256) {
-    T acc = 0;
-    for (int c = 0; c < M; c++) acc += Ui[r + (long long)c * M] * sx[c];
-    y[offX[i] + r] = acc;
-  }
-}
-
 // Scratch budget (elements) of one off-diagonal chunk of step 5.
 constexpr size_t PAIR_BUDGET = size_t(1) << 28;
 
@@ -136,9 +64,14 @@ struct UlvImpl : UlvBase {
     std::vector<long long> offRR;
     DevBuf<int> info;
     int info_sum = 0;
+    // substitution task lists: x = V y per box; sr terms A_ij[:s_i, s_j:] x_j[s_j:]
+    // (forward) and rs terms A_ij[s_i:, :s_j] w_j (backward) per near pair
+    PairList v_pl, fwd_pl, bwd_pl;
+    DevBuf<long long> d_toffU, d_toffB;
+    bool vecM = false;
   };
   std::vector<Lv> lv;
-  DevBuf<T> vin;
+  DevBuf<T> vin, tbuf;
 
   explicit UlvImpl(mxg_h2_s* hh) : h(hh) {
     hipStream_t s = h->ctx->stream;
@@ -163,7 +96,42 @@ struct UlvImpl : UlvBase {
       F.info.alloc_zero(std::max<long long>(H.nb, 1), s);
     }
     vin.alloc(h->n);
+    size_t tmax = 1;
+    for (long long l = 0; l <= h->L; l++) tmax = std::max(tmax, build_solve_lists(l));
+    tbuf.alloc(tmax);
     MXG_HIP(hipStreamSynchronize(s));
+  }
+
+  // Task lists of the substitution at level l; returns the term scratch it needs.
+  size_t build_solve_lists(long long l) {
+    hipStream_t s = h->ctx->stream;
+    Level& H = h->lv[l];
+    Lv& F = lv[l];
+    const long long nb = H.nb;
+    std::vector<PairTask> vt, ft, bt;
+    std::vector<long long> toffU(H.nrp[nb] + 1, 0), toffB(H.nrp[nb] + 1, 0);
+    F.vecM = true;
+    for (long long i = 0; i < nb; i++) {
+      const long long M = H.dims[i];
+      if (M % (16 / (long long)sizeof(T))) F.vecM = false;
+      if (M > 0) vt.push_back(PairTask{H.offQ[i], H.offX[i], H.offX[i], (int)M, (int)M, (int)M, 0});
+      const long long si = H.ranks[i];
+      for (long long p = H.nrp[i]; p < H.nrp[i + 1]; p++) {
+        const long long j = H.ncol[p];
+        const long long Mj = H.dims[j], sj = H.ranks[j];
+        toffU[p + 1] = toffU[p] + si;
+        toffB[p + 1] = toffB[p] + (M - si);
+        if (si > 0) ft.push_back(PairTask{H.offA[p] + sj * M, H.offX[j] + sj, toffU[p], (int)si, (int)(Mj - sj), (int)M, 0});
+        if (M - si > 0) bt.push_back(PairTask{H.offA[p] + si, H.offZ[j], toffB[p], (int)(M - si), (int)sj, (int)M, 0});
+      }
+    }
+    pair_list_build(F.v_pl, vt, H.offQ[nb], H.lenX, H.lenX, s);
+    pair_list_build(F.fwd_pl, ft, (long long)F.A.n, H.lenX, toffU.back(), s);
+    pair_list_build(F.bwd_pl, bt, (long long)F.A.n, H.lenZ, toffB.back(), s);
+    F.d_toffU.upload(toffU, s);
+    F.d_toffB.upload(toffB, s);
+    MXG_HIP(hipStreamSynchronize(s));
+    return (size_t)std::max(toffU.back(), toffB.back());
   }
 
   void factorize() override {
@@ -407,12 +375,10 @@ struct UlvImpl : UlvBase {
       Lv& F = lv[l];
       const T* y = (l == L) ? v : lv[l + 1].Z.p;
       ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_FWD_LEAF : -1);
-      hipLaunchKernelGGL(k_fwd1<T>, dim3((unsigned)H.nb), dim3(256), 0, s, H.d_dims.p, H.d_offQ.p, H.d_offX.p, F.V.p, y, F.X.p);
-      MXG_HIP(hipGetLastError());
+      pair_gemv<T>(F.v_pl, F.V.p, y, F.X.p, s);
       if (H.lenZ > 0) {
-        hipLaunchKernelGGL(k_fwd2<T>, dim3((unsigned)H.nb), dim3(256), 0, s, H.d_dims.p, H.d_ranks.p, H.d_offX.p, H.d_offZ.p,
-                           H.d_nrp.p, H.d_ncol.p, H.d_offA.p, F.A.p, F.X.p, F.Z.p);
-        MXG_HIP(hipGetLastError());
+        pair_gemv<T>(F.fwd_pl, F.A.p, F.X.p, tbuf.p, s);
+        comb_fwd<T>(H, F.d_toffU.p, F.X.p, tbuf.p, F.Z.p, s);
       }
     }
     for (long long l = 0; l <= L; l++) {
@@ -420,11 +386,9 @@ struct UlvImpl : UlvBase {
       Lv& F = lv[l];
       // w of level l is the output of level l - 1 (stored in this level's Z buffer)
       T* out = (l == L) ? v : lv[l + 1].Z.p;
-      size_t shm = std::max<long long>(H.maxM, 1) * sizeof(T);
       ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_BWD_LEAF : -1);
-      hipLaunchKernelGGL(k_bwd<T>, dim3((unsigned)H.nb), dim3(256), shm, s, H.d_dims.p, H.d_ranks.p, H.d_offQ.p, H.d_offX.p,
-                         H.d_offZ.p, H.d_nrp.p, H.d_ncol.p, H.d_offA.p, F.A.p, F.U.p, F.X.p, F.Z.p, out);
-      MXG_HIP(hipGetLastError());
+      pair_gemv<T>(F.bwd_pl, F.A.p, F.Z.p, tbuf.p, s);
+      comb_bwd<T>(H, F.vecM, F.d_toffB.p, F.U.p, F.X.p, F.Z.p, tbuf.p, out, s);
     }
     if constexpr (sizeof(T) == 8)
       MXG_HIP(hipMemcpyAsync(x, v, h->n * sizeof(double), hipMemcpyDeviceToDevice, s));
