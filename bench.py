@@ -96,10 +96,7 @@ def main():
         for k in range(args.warmup):
             solve(rhs[k], work, inner_prec)
         ctx.synchronize()
-        N.check_gpu(N.gpu().mxg_profile_reset(ctx.handle))
-        N.check_gpu(N.gpu().mxg_profile_enable(ctx.handle, 1))
         lat, cycles, inners, res = [], [], [], []
-        ctx.synchronize()
         t_start = time.perf_counter()
         for k in range(args.steps):
             ts = time.perf_counter()
@@ -110,6 +107,13 @@ def main():
             inners.append(inner)
         ctx.synchronize()
         elapsed = time.perf_counter() - t_start
+        # kernel timing with HIP events, in a separate pass over the same
+        # right-hand sides (the events stay out of the timed region)
+        N.check_gpu(N.gpu().mxg_profile_reset(ctx.handle))
+        N.check_gpu(N.gpu().mxg_profile_enable(ctx.handle, 1))
+        for k in range(min(args.steps, 3)):
+            solve(rhs[args.warmup + k], work, inner_prec)
+        ctx.synchronize()
         N.check_gpu(N.gpu().mxg_profile_enable(ctx.handle, 0))
         kern = {}
         import ctypes as C
@@ -134,13 +138,16 @@ def main():
     fp64 = run_path(N.MXG_FP64, N.MXG_FP64, 0)
     mixed = run_path(N.MXG_FP32, N.MXG_FP32, 1)
 
-    # roofline of the dominant kernel of the mixed path: the FP32 near-field
-    # product of the leaf level (every near block once, x read, y written)
+    # roofline of the dominant kernel of the mixed path: the FP32 product
+    # launch of the matvec (every leaf near block and every coupling block of
+    # every level once, the input x and every level's skeleton weights z read)
     h = N.h2().mxc_solver_device(A.handle)
     near_elems = int(N.gpu().mxg_h2_level_size(h, L, 0))
-    bytes_leaf_fp32 = 4 * (near_elems + 2 * n)
+    coupling_elems = sum(int(N.gpu().mxg_h2_level_size(h, l, 3)) for l in range(L + 1))
+    skeleton = sum(int(A.dims(l)[1].sum()) for l in range(L + 1))
+    bytes_prod_fp32 = 4 * (near_elems + coupling_elems + n + skeleton)
     km = mixed["kernels"]["matvec_leaf_fp32"]
-    achieved = bytes_leaf_fp32 / (km["avg_ms"] * 1e-3) / 1e9 if km["avg_ms"] else None
+    achieved = bytes_prod_fp32 / (km["avg_ms"] * 1e-3) / 1e9 if km["avg_ms"] else None
     sizes = []
     for l in range(L + 1):
         d, r = A.dims(l)
@@ -185,13 +192,13 @@ def main():
         "kernels_mixed": mixed["kernels"],
         "operator_levels": sizes,
         "roofline": {
-            "kernel": "k_pair_gemv<float, 4> over the leaf near blocks (FP32 near field of the matvec)",
+            "kernel": "k_pair_gemv<float>: the matvec's product launch (leaf near blocks + couplings of every level, FP32)",
             "bound": "hbm",
             "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "algorithmic_bytes_per_launch": bytes_leaf_fp32,
+            "algorithmic_bytes_per_launch": bytes_prod_fp32,
             "traffic": None,
         },
     }

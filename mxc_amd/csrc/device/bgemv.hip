@@ -39,24 +39,21 @@ __device__ inline void wg_gemv_pass(const T* __restrict__ B, long long ld, int n
   for (int q = 0; q < QN; q++) acc[q] = VT(0);
   const int nrows = min(ROWS, n - r0);
   int c = wave;
-  for (; c + 4 < m; c += 8) {
-    const T v0 = vfun(c), v1 = vfun(c + 4);
-    const T* col0 = B + (long long)c * ld + r0;
-    const T* col1 = col0 + 4 * ld;
-    VT a0[QN], a1[QN];
+  for (; c + 12 < m; c += 16) {
+    T vv[4];
+    VT a[4][QN];
 #pragma unroll
-    for (int q = 0; q < QN; q++) {
-      const int r = (q * 64 + lane) * V;
-      if (r < nrows) {
-        a0[q] = *reinterpret_cast<const VT*>(col0 + r);
-        a1[q] = *reinterpret_cast<const VT*>(col1 + r);
-      } else {
-        a0[q] = VT(0);
-        a1[q] = VT(0);
+    for (int u = 0; u < 4; u++) {
+      vv[u] = vfun(c + 4 * u);
+      const T* col = B + (long long)(c + 4 * u) * ld + r0;
+#pragma unroll
+      for (int q = 0; q < QN; q++) {
+        const int r = (q * 64 + lane) * V;
+        a[u][q] = (r < nrows) ? *reinterpret_cast<const VT*>(col + r) : VT(0);
       }
     }
 #pragma unroll
-    for (int q = 0; q < QN; q++) acc[q] += a0[q] * v0 + a1[q] * v1;
+    for (int q = 0; q < QN; q++) acc[q] += (a[0][q] * vv[0] + a[1][q] * vv[1]) + (a[2][q] * vv[2] + a[3][q] * vv[3]);
   }
   for (; c < m; c += 4) {
     const T v0 = vfun(c);
@@ -77,23 +74,74 @@ __device__ inline void wg_gemv_pass(const T* __restrict__ B, long long ld, int n
   __syncthreads();
 }
 
+// Short columns (n <= 64 V): a column takes lpc = 2^ceil(log2(n / V)) lanes
+// and each wave works on 64 / lpc columns at once, so no lane idles for want
+// of rows. The 4 * 64 / lpc partial columns are summed through LDS.
+template <class T, int V, class VF, class EF>
+__device__ inline void wg_gemv_narrow(const T* __restrict__ B, long long ld, int n, int m, VF vfun, EF emit, T* red) {
+  typedef typename Vec<T, V>::type VT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nv = (n + V - 1) / V;
+  int lpc = 1;
+  while (lpc < nv) lpc <<= 1;
+  const int G = 64 / lpc, g = lane / lpc, lr = lane % lpc;
+  const int r = lr * V;
+  const bool active = lr < nv;
+  const int stride = 4 * G;
+  VT acc = VT(0);
+  int c = wave * G + g;
+  for (; c + 3 * stride < m; c += 4 * stride) {
+    T vv[4];
+    VT a[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      vv[u] = vfun(c + u * stride);
+      a[u] = active ? *reinterpret_cast<const VT*>(B + (long long)(c + u * stride) * ld + r) : VT(0);
+    }
+    acc += (a[0] * vv[0] + a[1] * vv[1]) + (a[2] * vv[2] + a[3] * vv[3]);
+  }
+  for (; c < m; c += stride)
+    if (active) acc += *reinterpret_cast<const VT*>(B + (long long)c * ld + r) * vfun(c);
+  const int W = lpc * V;  // row stride of the partial columns in LDS
+  if (active) *reinterpret_cast<VT*>(red + (wave * G + g) * W + r) = acc;
+  else if (r < W) *reinterpret_cast<VT*>(red + (wave * G + g) * W + r) = VT(0);
+  __syncthreads();
+  const int slots = 4 * G;
+  for (int rr = threadIdx.x; rr < n; rr += 256) {
+    T sum = T(0);
+    for (int k = 0; k < slots; k++) sum += red[k * W + rr];
+    emit(rr, sum);
+  }
+  __syncthreads();
+}
+
 template <class T, int V, class VF, class EF>
 __device__ inline void wg_gemv(const T* __restrict__ B, long long ld, int n, int m, VF vfun, EF emit, T* red) {
+  if (n <= 64 * V) {
+    wg_gemv_narrow<T, V>(B, ld, n, m, vfun, emit, red);
+    return;
+  }
   for (int r0 = 0; r0 < n; r0 += 64 * V * QN) wg_gemv_pass<T, V>(B, ld, n, m, r0, vfun, emit, red);
 }
 
-template <class T, int V>
-__global__ void __launch_bounds__(256) k_pair_gemv(const PairTask* __restrict__ tasks, const T* __restrict__ B,
-                                                   const T* __restrict__ v, T* __restrict__ out) {
-  __shared__ __align__(16) T red[4 * 64 * V * QN];
-  const PairTask t = tasks[blockIdx.x];
-  const T* vb = v + t.v;
-  T* ob = out + t.out;
+template <class T>
+__global__ void __launch_bounds__(256) k_pair_gemv(const PairJob* __restrict__ jobs) {
+  constexpr int VW = 16 / (int)sizeof(T);
+  __shared__ __align__(16) T red[4 * 64 * VW * QN];
+  const PairJob t = jobs[blockIdx.x];
+  const T* B = static_cast<const T*>(t.B);
+  const T* vb = static_cast<const T*>(t.v);
+  T* ob = static_cast<T*>(t.out);
   if (t.m == 0) {
     for (int r = threadIdx.x; r < t.n; r += 256) ob[r] = T(0);
     return;
   }
-  wg_gemv<T, V>(B + t.b, t.ld, t.n, t.m, [&](int c) { return vb[c]; }, [&](int r, T s) { ob[r] = s; }, red);
+  auto vf = [&](int c) { return vb[c]; };
+  auto ef = [&](int r, T s) { ob[r] = s; };
+  if (t.vec)
+    wg_gemv<T, VW>(B, t.ld, t.n, t.m, vf, ef, red);
+  else
+    wg_gemv<T, 1>(B, t.ld, t.n, t.m, vf, ef, red);
 }
 
 // Downward pass of one level: w_i = w_in_i + sum_far t_p (k_i entries), then
@@ -181,40 +229,37 @@ constexpr int vwidth() {
 }  // namespace
 
 template <class T>
-bool pair_tasks_vectorizable(const std::vector<PairTask>& tasks) {
+void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& tasks, const T* B, long long nB, const T* v,
+                      long long nv, T* out, long long nout) {
   constexpr long long W = vwidth<T>();
-  for (const PairTask& t : tasks)
-    if (t.n % W || t.ld % W || t.b % W) return false;
-  return true;
-}
-template bool pair_tasks_vectorizable<float>(const std::vector<PairTask>&);
-template bool pair_tasks_vectorizable<double>(const std::vector<PairTask>&);
-
-void pair_list_build(PairList& pl, const std::vector<PairTask>& tasks, long long nB, long long nv, long long nout,
-                     hipStream_t s) {
   for (const PairTask& t : tasks) {
     MXG_REQUIRE(t.n >= 0 && t.m >= 0 && t.ld >= std::max(t.n, 1) && t.b >= 0 && t.v >= 0 && t.out >= 0, "bad pair task");
     MXG_REQUIRE(t.out + t.n <= nout && t.v + t.m <= nv, "pair task outside its vectors");
     MXG_REQUIRE(t.n == 0 || t.m == 0 || t.b + (long long)(t.m - 1) * t.ld + t.n <= nB, "pair task outside its matrix");
+    const T* b = B + t.b;
+    const int vec = ((reinterpret_cast<uintptr_t>(b) % 16) == 0 && t.ld % W == 0 && t.n % W == 0) ? 1 : 0;
+    jobs.push_back(PairJob{b, v + t.v, out + t.out, t.n, t.m, t.ld, vec});
   }
-  pl.count = (long long)tasks.size();
-  pl.vec[0] = pair_tasks_vectorizable<double>(tasks);
-  pl.vec[1] = pair_tasks_vectorizable<float>(tasks);
-  if (pl.count) pl.tasks.upload(tasks, s);
+}
+template void pair_jobs_append<float>(std::vector<PairJob>&, const std::vector<PairTask>&, const float*, long long,
+                                      const float*, long long, float*, long long);
+template void pair_jobs_append<double>(std::vector<PairJob>&, const std::vector<PairTask>&, const double*, long long,
+                                       const double*, long long, double*, long long);
+
+void pair_list_upload(PairList& pl, const std::vector<PairJob>& jobs, hipStream_t s) {
+  MXG_REQUIRE(jobs.size() < (1ull << 31), "pair list too long");
+  pl.count = (long long)jobs.size();
+  if (pl.count) pl.jobs.upload(jobs, s);
 }
 
 template <class T>
-void pair_gemv(const PairList& pl, const T* B, const T* v, T* out, hipStream_t s) {
+void pair_gemv(const PairList& pl, hipStream_t s) {
   if (pl.count == 0) return;
-  MXG_REQUIRE(pl.count < (1ll << 31), "pair list too long");
-  if (pl.vec[sizeof(T) == 4 ? 1 : 0])
-    hipLaunchKernelGGL((k_pair_gemv<T, vwidth<T>()>), dim3((unsigned)pl.count), dim3(256), 0, s, pl.tasks.p, B, v, out);
-  else
-    hipLaunchKernelGGL((k_pair_gemv<T, 1>), dim3((unsigned)pl.count), dim3(256), 0, s, pl.tasks.p, B, v, out);
+  hipLaunchKernelGGL(k_pair_gemv<T>, dim3((unsigned)pl.count), dim3(256), 0, s, pl.jobs.p);
   MXG_HIP(hipGetLastError());
 }
-template void pair_gemv<float>(const PairList&, const float*, const float*, float*, hipStream_t);
-template void pair_gemv<double>(const PairList&, const double*, const double*, double*, hipStream_t);
+template void pair_gemv<float>(const PairList&, hipStream_t);
+template void pair_gemv<double>(const PairList&, hipStream_t);
 
 template <class T>
 void comb_down(const Level& Lv, bool vec, const long long* d_toffF, const long long* d_toffN, const T* Q, const T* w_in,

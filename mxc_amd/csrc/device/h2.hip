@@ -424,8 +424,8 @@ void upload_indices(mxg_h2_s* h, Level& Lv) {
 
 }  // namespace
 
-// Matvec task lists of a constructed level and the term offsets of its
-// downward pass; sizes the shared term scratch.
+// Matvec tasks of a constructed level and the term offsets of its downward
+// pass (the device jobs are made per precision on first use).
 void build_matvec_lists(mxg_h2_s* h, Level& Lv, bool leaf) {
   hipStream_t s = h->ctx->stream;
   const long long nb = Lv.nb;
@@ -437,33 +437,33 @@ void build_matvec_lists(mxg_h2_s* h, Level& Lv, bool leaf) {
   hipLaunchKernelGGL(k_basis_transpose, dim3((unsigned)nb), dim3(256), 0, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p, d_offQT.p,
                      Lv.Q.p, Lv.QT.p);
   MXG_HIP(hipGetLastError());
-  std::vector<PairTask> up, far, near;
+  Lv.up_t.clear();
+  Lv.far_t.clear();
+  Lv.near_t.clear();
   for (long long i = 0; i < nb; i++)
     if (Lv.ranks[i] > 0)
-      up.push_back(PairTask{Lv.offQT[i], Lv.offX[i], Lv.offZ[i], (int)Lv.ranks[i], (int)Lv.dims[i], (int)Lv.ranks[i], 0});
+      Lv.up_t.push_back(PairTask{Lv.offQT[i], Lv.offX[i], Lv.offZ[i], (int)Lv.ranks[i], (int)Lv.dims[i], (int)Lv.ranks[i], 0});
   Lv.toffF.assign(Lv.frp[nb] + 1, 0);
   for (long long i = 0; i < nb; i++)
     for (long long p = Lv.frp[i]; p < Lv.frp[i + 1]; p++) {
       const long long j = Lv.fcol[p];
       Lv.toffF[p + 1] = Lv.toffF[p] + Lv.ranks[i];
       if (Lv.ranks[i] > 0)
-        far.push_back(PairTask{Lv.offC[p], Lv.offZ[j], Lv.toffF[p], (int)Lv.ranks[i], (int)Lv.ranks[j], (int)Lv.ranks[i], 0});
+        Lv.far_t.push_back(PairTask{Lv.offC[p], Lv.offZ[j], Lv.toffF[p], (int)Lv.ranks[i], (int)Lv.ranks[j], (int)Lv.ranks[i], 0});
     }
   Lv.lenTF = Lv.toffF.back();
   Lv.toffN.assign(Lv.nrp[nb] + 1, 0);
+  Lv.lenTN = 0;
   if (leaf) {
     for (long long i = 0; i < nb; i++)
       for (long long p = Lv.nrp[i]; p < Lv.nrp[i + 1]; p++) {
         const long long j = Lv.ncol[p];
         Lv.toffN[p + 1] = Lv.toffN[p] + Lv.dims[i];
         if (Lv.dims[i] > 0)
-          near.push_back(PairTask{Lv.offA[p], Lv.offX[j], Lv.toffN[p], (int)Lv.dims[i], (int)Lv.dims[j], (int)Lv.dims[i], 0});
+          Lv.near_t.push_back(PairTask{Lv.offA[p], Lv.offX[j], Lv.toffN[p], (int)Lv.dims[i], (int)Lv.dims[j], (int)Lv.dims[i], 0});
       }
     Lv.lenTN = Lv.toffN.back();
   }
-  pair_list_build(Lv.up_pl, up, Lv.offQT[nb], Lv.lenX, Lv.lenZ, s);
-  pair_list_build(Lv.far_pl, far, (long long)Lv.C.n, Lv.lenZ, Lv.lenTF, s);
-  if (leaf) pair_list_build(Lv.near_pl, near, (long long)Lv.A.n, h->n, Lv.lenTN, s);
   Lv.d_toffF.upload(Lv.toffF, s);
   Lv.d_toffN.upload(Lv.toffN, s);
   Lv.vecQ[0] = Lv.vecQ[1] = true;
@@ -471,60 +471,105 @@ void build_matvec_lists(mxg_h2_s* h, Level& Lv, bool leaf) {
     if (Lv.dims[i] % 2) Lv.vecQ[0] = false;
     if (Lv.dims[i] % 4) Lv.vecQ[1] = false;
   }
-  h->t64.reserve((size_t)std::max<long long>(Lv.lenTF + Lv.lenTN, 1));
   MXG_HIP(hipStreamSynchronize(s));
 }
 
+namespace {
+
+template <class T>
+struct Ops;
+template <>
+struct Ops<double> {
+  static const double* Q(Level& L) { return L.Q.p; }
+  static const double* QT(Level& L) { return L.QT.p; }
+  static const double* C(Level& L) { return L.C.p; }
+  static const double* A(Level& L) { return L.A.p; }
+  static double* Z(mxg_h2_s* h, long long l) { return h->Z[l].p; }
+  static double* W(mxg_h2_s* h, long long l) { return h->W[l].p; }
+  static double* x(mxg_h2_s* h) { return h->x64.p; }
+  static DevBuf<double>& t(mxg_h2_s* h) { return h->t64; }
+};
+template <>
+struct Ops<float> {
+  static const float* Q(Level& L) { return L.Q32.p; }
+  static const float* QT(Level& L) { return L.QT32.p; }
+  static const float* C(Level& L) { return L.C32.p; }
+  static const float* A(Level& L) { return L.A32.p; }
+  static float* Z(mxg_h2_s* h, long long l) { return h->Z32[l].p; }
+  static float* W(mxg_h2_s* h, long long l) { return h->W32[l].p; }
+  static float* x(mxg_h2_s* h) { return h->x32.p; }
+  static DevBuf<float>& t(mxg_h2_s* h) { return h->t32; }
+};
+
+// Device jobs of one precision: the upward pass per level, and one list with
+// the far terms of every level and the near terms of the leaf level.
+template <class T>
+void ensure_matvec_jobs(mxg_h2_s* h) {
+  constexpr int pi = sizeof(T) == 8 ? 0 : 1;
+  if (h->jobs_ready[pi]) return;
+  hipStream_t s = h->ctx->stream;
+  const long long L = h->L;
+  long long tlen = 0;
+  for (long long l = 0; l <= L; l++) {
+    h->lv[l].tbaseF = tlen;
+    tlen += h->lv[l].lenTF;
+  }
+  h->lv[L].tbaseN = tlen;
+  tlen += h->lv[L].lenTN;
+  DevBuf<T>& tb = Ops<T>::t(h);
+  tb.alloc((size_t)std::max<long long>(tlen, 1));
+  if constexpr (pi == 0) h->x64.reserve((size_t)h->n);
+  std::vector<PairJob> prod;
+  for (long long l = 0; l <= L; l++) {
+    Level& Lv = h->lv[l];
+    std::vector<PairJob> up;
+    const T* xin = (l == L) ? Ops<T>::x(h) : Ops<T>::Z(h, l + 1);
+    pair_jobs_append<T>(up, Lv.up_t, Ops<T>::QT(Lv), (long long)Lv.offQT[Lv.nb], xin, Lv.lenX, Ops<T>::Z(h, l),
+                        std::max<long long>(Lv.lenZ, 1));
+    pair_list_upload(Lv.up_pl[pi], up, s);
+    pair_jobs_append<T>(prod, Lv.far_t, Ops<T>::C(Lv), (long long)Lv.C.n, Ops<T>::Z(h, l), std::max<long long>(Lv.lenZ, 1),
+                        tb.p + Lv.tbaseF, Lv.lenTF);
+  }
+  Level& Lf = h->lv[L];
+  pair_jobs_append<T>(prod, Lf.near_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, tb.p + Lf.tbaseN, Lf.lenTN);
+  pair_list_upload(h->prod_pl[pi], prod, s);
+  MXG_HIP(hipStreamSynchronize(s));
+  h->jobs_ready[pi] = true;
+}
+
+}  // namespace
+
+// y = A x on the device. x and y are FP64 (T = double) or the FP32 copies
+// h->x32 / h->y32 (T = float); x is staged into the internal input buffer.
 template <class T>
 void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y) {
   hipStream_t s = h->ctx->stream;
   const long long L = h->L;
   constexpr bool f64 = sizeof(T) == 8;
-  auto Zp = [&](long long l) -> T* {
-    if constexpr (f64) return reinterpret_cast<T*>(h->Z[l].p); else return reinterpret_cast<T*>(h->Z32[l].p);
-  };
-  auto Wp = [&](long long l) -> T* {
-    if constexpr (f64) return reinterpret_cast<T*>(h->W[l].p); else return reinterpret_cast<T*>(h->W32[l].p);
-  };
-  auto Qp = [&](Level& Lv) -> const T* {
-    if constexpr (f64) return reinterpret_cast<const T*>(Lv.Q.p); else return reinterpret_cast<const T*>(Lv.Q32.p);
-  };
-  auto QTp = [&](Level& Lv) -> const T* {
-    if constexpr (f64) return reinterpret_cast<const T*>(Lv.QT.p); else return reinterpret_cast<const T*>(Lv.QT32.p);
-  };
-  auto Cp = [&](Level& Lv) -> const T* {
-    if constexpr (f64) return reinterpret_cast<const T*>(Lv.C.p); else return reinterpret_cast<const T*>(Lv.C32.p);
-  };
-  auto Ap = [&](Level& Lv) -> const T* {
-    if constexpr (f64) return reinterpret_cast<const T*>(Lv.A.p); else return reinterpret_cast<const T*>(Lv.A32.p);
-  };
-  T* tbuf;
-  if constexpr (f64) tbuf = h->t64.p; else tbuf = h->t32.p;
+  constexpr int pi = f64 ? 0 : 1;
+  ensure_matvec_jobs<T>(h);
+  T* xb = Ops<T>::x(h);
+  if (x != xb) MXG_HIP(hipMemcpyAsync(xb, x, h->n * sizeof(T), hipMemcpyDeviceToDevice, s));
+  T* tb = Ops<T>::t(h).p;
   // upward pass: z_i = Q_i[:, :k]^T x_i
   for (long long l = L; l >= 0; l--) {
-    Level& Lv = h->lv[l];
-    if (Lv.lenZ == 0) continue;
-    const T* xin = (l == L) ? x : Zp(l + 1);
+    if (h->lv[l].lenZ == 0) continue;
     ProfScope prof(h->ctx, MXG_PROF_MATVEC_UPPER);
-    pair_gemv<T>(Lv.up_pl, QTp(Lv), xin, Zp(l), s);
+    pair_gemv<T>(h->lv[l].up_pl[pi], s);
   }
-  // downward pass: couplings into w, then y = Q w (+ near field at the leaf)
+  // every far term of every level and the near terms of the leaf, one launch
+  {
+    ProfScope prof(h->ctx, f64 ? MXG_PROF_MATVEC_LEAF_FP64 : MXG_PROF_MATVEC_LEAF_FP32);
+    pair_gemv<T>(h->prod_pl[pi], s);
+  }
+  // downward pass: w = w_in + far terms, y = Q w (+ near terms at the leaf)
   for (long long l = 0; l <= L; l++) {
     Level& Lv = h->lv[l];
     const bool leaf = (l == L);
-    T* out = leaf ? y : Wp(l + 1);
-    T* tF = tbuf;
-    T* tN = tbuf + Lv.lenTF;
-    {
-      ProfScope prof(h->ctx, MXG_PROF_MATVEC_UPPER);
-      pair_gemv<T>(Lv.far_pl, Cp(Lv), Zp(l), tF, s);
-    }
-    if (leaf) {
-      ProfScope prof(h->ctx, f64 ? MXG_PROF_MATVEC_LEAF_FP64 : MXG_PROF_MATVEC_LEAF_FP32);
-      pair_gemv<T>(Lv.near_pl, Ap(Lv), x, tN, s);
-    }
+    T* out = leaf ? y : Ops<T>::W(h, l + 1);
     ProfScope prof(h->ctx, MXG_PROF_MATVEC_UPPER);
-    comb_down<T>(Lv, Lv.vecQ[f64 ? 0 : 1], Lv.d_toffF.p, Lv.d_toffN.p, Qp(Lv), Wp(l), tF, leaf ? tN : nullptr, out, s);
+    comb_down<T>(Lv, Lv.vecQ[pi], Lv.d_toffF.p, Lv.d_toffN.p, Ops<T>::Q(Lv), Ops<T>::W(h, l), tb + Lv.tbaseF,
+                 leaf ? tb + Lv.tbaseN : nullptr, out, s);
   }
 }
 
@@ -559,7 +604,7 @@ void make_shadow32(mxg_h2_s* h) {
   }
   h->x32.alloc(h->n);
   h->y32.alloc(h->n);
-  h->t32.alloc(std::max<size_t>(h->t64.n, 1));
+  h->jobs_ready[1] = false;
   MXG_HIP(hipStreamSynchronize(s));
   h->shadow32 = true;
 }
@@ -744,6 +789,7 @@ int mxg_h2_construct_level(mxg_h2_t h, long long l, long long nb, const long lon
     construct_coupling(h, Lv, orth);
     upload_indices(h, Lv);
     build_matvec_lists(h, Lv, !low);
+    h->jobs_ready[0] = h->jobs_ready[1] = false;
     h->Z[l].alloc_zero(std::max<long long>(Lv.lenZ, 1), s);
     h->W[l].alloc_zero(std::max<long long>(Lv.lenZ, 1), s);
     MXG_HIP(hipStreamSynchronize(s));

@@ -19,18 +19,23 @@ namespace mxg {
 
 // One dense product term t = B v of a block-sparse pass: B is an n x m
 // column-major (sub-)block with leading dimension ld; b, v and out are
-// element offsets from the base pointers given at launch.
+// element offsets into the matrix, input and output arrays of its pass.
 struct PairTask {
   long long b, v, out;
   int n, m, ld;
   int pad;
 };
-// A task list resident on the device. vec[0] / vec[1]: every block admits
-// 16-byte column loads in FP64 / FP32 (rows, ld and offset multiples of 2 / 4).
+// The device form of a task: absolute addresses, and vec = 1 when the block
+// admits 16-byte column loads (address, rows and ld multiples of 16 bytes).
+struct PairJob {
+  const void* B;
+  const void* v;
+  void* out;
+  int n, m, ld, vec;
+};
 struct PairList {
-  DevBuf<PairTask> tasks;
+  DevBuf<PairJob> jobs;
   long long count = 0;
-  bool vec[2] = {false, false};
 };
 
 struct Level {
@@ -49,14 +54,16 @@ struct Level {
   std::vector<int> up_ld;
   // FP32 copies of the operator data (inner matvec of mixed precision GMRES)
   DevBuf<float> A32, Q32, C32;
-  // matvec task lists: up z_i = QT_i x_i (QT_i = Q_i[:, :k]^T, k x M), far
+  // matvec tasks: up z_i = QT_i x_i (QT_i = Q_i[:, :k]^T, k x M), far
   // t_p = C_p z_j, leaf near u_p = A_p x_j; toffF / toffN: term offsets
+  // within the level's share of the term scratch (tbaseF / tbaseN)
   std::vector<long long> offQT, toffF, toffN;
+  std::vector<PairTask> up_t, far_t, near_t;
   DevBuf<double> QT;
   DevBuf<float> QT32;
-  PairList up_pl, far_pl, near_pl;
+  PairList up_pl[2];  // [0] FP64, [1] FP32
   DevBuf<long long> d_toffF, d_toffN;
-  long long lenTF = 0, lenTN = 0;
+  long long lenTF = 0, lenTN = 0, tbaseF = 0, tbaseN = 0;
   bool vecQ[2] = {false, false};  // every M_i admits 16-byte loads of Q columns (FP64 / FP32)
 };
 
@@ -73,9 +80,12 @@ struct mxg_h2_s {
   std::vector<mxg::DevBuf<float>> Z32, W32;
   mxg::DevBuf<float> x32, y32;
   bool shadow32 = false;
-  // scratch for the product terms of one level's passes
-  mxg::DevBuf<double> t64;
+  // matvec: input copy, product terms of every level, and the one launch that
+  // computes all far terms and the leaf near terms ([0] FP64, [1] FP32)
+  mxg::DevBuf<double> x64, t64;
   mxg::DevBuf<float> t32;
+  mxg::PairList prod_pl[2];
+  bool jobs_ready[2] = {false, false};
 };
 
 namespace mxg {
@@ -87,15 +97,15 @@ template <class T>
 void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y);
 
 // ---- block-sparse products (bgemv.hip) ----
+// Appends the jobs of tasks over the arrays B (nB elements), v (nv) and out
+// (nout), after checking every task against those sizes.
 template <class T>
-bool pair_tasks_vectorizable(const std::vector<PairTask>& tasks);
-// Uploads tasks into pl and sets its alignment flags, after checking every
-// task against the sizes (elements) of the matrix, input and output buffers.
-void pair_list_build(PairList& pl, const std::vector<PairTask>& tasks, long long nB, long long nv, long long nout,
-                     hipStream_t s);
-// out[t.out + r] = sum_c B[t.b + r + c*ld] v[t.v + c] for every task
+void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& tasks, const T* B, long long nB, const T* v,
+                      long long nv, T* out, long long nout);
+void pair_list_upload(PairList& pl, const std::vector<PairJob>& jobs, hipStream_t s);
+// out[r] = sum_c B[r + c*ld] v[c] for every job, one workgroup per job
 template <class T>
-void pair_gemv(const PairList& pl, const T* B, const T* v, T* out, hipStream_t s);
+void pair_gemv(const PairList& pl, hipStream_t s);
 // w_i = w_in_i + sum_far tF, y_i = Q_i[:, :k] w_i (+ sum_near tN when tN != nullptr)
 template <class T>
 void comb_down(const Level& Lv, bool vec, const long long* d_toffF, const long long* d_toffN, const T* Q, const T* w_in,

@@ -96,14 +96,24 @@ struct UlvImpl : UlvBase {
       F.info.alloc_zero(std::max<long long>(H.nb, 1), s);
     }
     vin.alloc(h->n);
-    size_t tmax = 1;
-    for (long long l = 0; l <= h->L; l++) tmax = std::max(tmax, build_solve_lists(l));
-    tbuf.alloc(tmax);
+    long long tmax = 1;
+    for (long long l = 0; l <= h->L; l++) {
+      const Level& H = h->lv[l];
+      long long u = 0, b = 0;
+      for (long long i = 0; i < H.nb; i++) {
+        const long long cnt = H.nrp[i + 1] - H.nrp[i];
+        u += cnt * H.ranks[i];
+        b += cnt * (H.dims[i] - H.ranks[i]);
+      }
+      tmax = std::max(tmax, std::max(u, b));
+    }
+    tbuf.alloc((size_t)tmax);
+    for (long long l = 0; l <= h->L; l++) build_solve_lists(l, tmax);
     MXG_HIP(hipStreamSynchronize(s));
   }
 
-  // Task lists of the substitution at level l; returns the term scratch it needs.
-  size_t build_solve_lists(long long l) {
+  // Jobs of the substitution at level l (terms in tbuf, tlen elements).
+  void build_solve_lists(long long l, long long tlen) {
     hipStream_t s = h->ctx->stream;
     Level& H = h->lv[l];
     Lv& F = lv[l];
@@ -125,13 +135,18 @@ struct UlvImpl : UlvBase {
         if (M - si > 0) bt.push_back(PairTask{H.offA[p] + si, H.offZ[j], toffB[p], (int)(M - si), (int)sj, (int)M, 0});
       }
     }
-    pair_list_build(F.v_pl, vt, H.offQ[nb], H.lenX, H.lenX, s);
-    pair_list_build(F.fwd_pl, ft, (long long)F.A.n, H.lenX, toffU.back(), s);
-    pair_list_build(F.bwd_pl, bt, (long long)F.A.n, H.lenZ, toffB.back(), s);
+    const T* y = (l == h->L) ? vin.p : lv[l + 1].Z.p;
+    const long long ylen = (l == h->L) ? h->n : std::max<long long>(h->lv[l + 1].lenZ, 1);
+    std::vector<PairJob> jv, jf, jb;
+    pair_jobs_append<T>(jv, vt, F.V.p, H.offQ[nb], y, ylen, F.X.p, std::max<long long>(H.lenX, 1));
+    pair_jobs_append<T>(jf, ft, F.A.p, (long long)F.A.n, F.X.p, std::max<long long>(H.lenX, 1), tbuf.p, tlen);
+    pair_jobs_append<T>(jb, bt, F.A.p, (long long)F.A.n, F.Z.p, std::max<long long>(H.lenZ, 1), tbuf.p, tlen);
+    pair_list_upload(F.v_pl, jv, s);
+    pair_list_upload(F.fwd_pl, jf, s);
+    pair_list_upload(F.bwd_pl, jb, s);
     F.d_toffU.upload(toffU, s);
     F.d_toffB.upload(toffB, s);
     MXG_HIP(hipStreamSynchronize(s));
-    return (size_t)std::max(toffU.back(), toffB.back());
   }
 
   void factorize() override {
@@ -373,11 +388,10 @@ struct UlvImpl : UlvBase {
     for (long long l = L; l >= 0; l--) {
       Level& H = h->lv[l];
       Lv& F = lv[l];
-      const T* y = (l == L) ? v : lv[l + 1].Z.p;
       ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_FWD_LEAF : -1);
-      pair_gemv<T>(F.v_pl, F.V.p, y, F.X.p, s);
+      pair_gemv<T>(F.v_pl, s);
       if (H.lenZ > 0) {
-        pair_gemv<T>(F.fwd_pl, F.A.p, F.X.p, tbuf.p, s);
+        pair_gemv<T>(F.fwd_pl, s);
         comb_fwd<T>(H, F.d_toffU.p, F.X.p, tbuf.p, F.Z.p, s);
       }
     }
@@ -387,7 +401,7 @@ struct UlvImpl : UlvBase {
       // w of level l is the output of level l - 1 (stored in this level's Z buffer)
       T* out = (l == L) ? v : lv[l + 1].Z.p;
       ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_BWD_LEAF : -1);
-      pair_gemv<T>(F.bwd_pl, F.A.p, F.Z.p, tbuf.p, s);
+      pair_gemv<T>(F.bwd_pl, s);
       comb_bwd<T>(H, F.vecM, F.d_toffB.p, F.U.p, F.X.p, F.Z.p, tbuf.p, out, s);
     }
     if constexpr (sizeof(T) == 8)
