@@ -10,7 +10,10 @@ solve: GMRES from b (U(-1, 1), a fresh seed per step) to
 ||b - A x|| / ||b|| <= 1e-10 on the FP64 H^2 operator, restart 30, with the
 factorization resident in HBM. The mixed path runs the ULV in FP32 and the
 inner operator applications in FP32 (FP64 Arnoldi, Givens, outer residual);
-the FP64 path runs everything in FP64. `value` is the mixed path's solves/s;
+the FP64 path runs everything in FP64. The mixed path's inner iterations
+apply the operator built at the looser tolerance INNER_EPS (GMRES iterative
+refinement: every cycle's residual uses the FP64 operator at OP_EPS).
+`value` is the mixed path's solves/s;
 the FP64 path is reported beside it. After the timed region every solution is
 checked with the matrix-free FP64 dense matvec.
 
@@ -38,6 +41,8 @@ def parse():
     ap.add_argument("--n", type=int, default=1 << 20, help="points (default 2^20)")
     ap.add_argument("--op-eps", type=float, default=1e-12)
     ap.add_argument("--sampling", type=int, default=2000)
+    ap.add_argument("--inner-eps", type=float, default=1e-7,
+                    help="tolerance of the operator the mixed path applies in its FP32 inner iterations (0: the main operator)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=16384, help="points of the CPU baseline sample")
     return ap.parse_args()
@@ -78,6 +83,11 @@ def main():
     M = H2.preconditioner(prob, 64, args.sampling, kernel, diag, alpha, theta, eps=1e-6)
     t_pc = time.time() - t0
     log(f"preconditioner built in {t_pc:.1f} s")
+    t0 = time.time()
+    A_in = H2.operator(prob, args.inner_eps, args.sampling, kernel, diag, alpha, theta) if args.inner_eps > 0 else None
+    t_in = time.time() - t0
+    if A_in is not None:
+        log(f"inner operator (eps {args.inner_eps:g}) built in {t_in:.1f} s")
     L = prob.levels
 
     rhs = [ctx.array(uniform_vector(n, -1.0, 1.0, 1000 + k)) for k in range(args.warmup + args.steps)]
@@ -86,7 +96,8 @@ def main():
 
     def solve(b, x, inner_prec):
         x.zero()
-        res, it, inner = A.gmres(M, b, x, tol, restart, max_cycles, N.GMRES_CGS2, inner_prec)
+        inner_op = A_in if inner_prec == N.MXG_FP32 else None
+        res, it, inner = A.gmres(M, b, x, tol, restart, max_cycles, N.GMRES_CGS2, inner_prec, inner_op)
         return res[-1], it, inner
 
     def run_path(factor_prec, inner_prec, prof_class):
@@ -141,19 +152,21 @@ def main():
     # roofline of the dominant kernel of the mixed path: the FP32 product
     # launch of the matvec (every leaf near block and every coupling block of
     # every level once, the input x and every level's skeleton weights z read)
-    h = N.h2().mxc_solver_device(A.handle)
+    Afp32 = A_in if A_in is not None else A
+    h = N.h2().mxc_solver_device(Afp32.handle)
     near_elems = int(N.gpu().mxg_h2_level_size(h, L, 0))
     coupling_elems = sum(int(N.gpu().mxg_h2_level_size(h, l, 3)) for l in range(L + 1))
-    skeleton = sum(int(A.dims(l)[1].sum()) for l in range(L + 1))
+    skeleton = sum(int(Afp32.dims(l)[1].sum()) for l in range(L + 1))
     bytes_prod_fp32 = 4 * (near_elems + coupling_elems + n + skeleton)
     km = mixed["kernels"]["matvec_leaf_fp32"]
     achieved = bytes_prod_fp32 / (km["avg_ms"] * 1e-3) / 1e9 if km["avg_ms"] else None
     sizes = []
+    hA = N.h2().mxc_solver_device(A.handle)
     for l in range(L + 1):
         d, r = A.dims(l)
         sizes.append({"level": l, "boxes": len(d), "sum_dims": int(d.sum()), "sum_ranks": int(r.sum()),
-                      "near_elems": int(N.gpu().mxg_h2_level_size(h, l, 0)),
-                      "coupling_elems": int(N.gpu().mxg_h2_level_size(h, l, 3)), "basis_elems": int(np.sum(d * r))})
+                      "near_elems": int(N.gpu().mxg_h2_level_size(hA, l, 0)),
+                      "coupling_elems": int(N.gpu().mxg_h2_level_size(hA, l, 3)), "basis_elems": int(np.sum(d * r))})
 
     value = args.steps / mixed["elapsed"]
     value64 = args.steps / fp64["elapsed"]
@@ -173,7 +186,8 @@ def main():
         "config": {
             "workload": "H2 GMRES solve, unit-sphere Yukawa N=2^20, leaf 256",
             "N": n, "leaf": leaf, "theta": theta, "kernel": "yukawa exp(-r)/r, K(0)=1e3",
-            "operator_eps": args.op_eps, "sampling": args.sampling, "precond_eps": 1e-6, "precond_max_rank": 64,
+            "operator_eps": args.op_eps, "inner_operator_eps": args.inner_eps, "sampling": args.sampling,
+            "precond_eps": 1e-6, "precond_max_rank": 64,
             "restart": restart, "tol": tol, "levels": L,
         },
         "p50_ms": round(1e3 * pct(mixed["lat"], 50), 3),
@@ -188,7 +202,7 @@ def main():
             "cycles": mixed["cycles"], "inner_iters": mixed["inner"], "h2_relres_max": max(mixed["h2_resid"]),
             "dense_relres_max": max(mixed["dense"]), "factor_s": round(mixed["factor_s"], 4),
         },
-        "setup_s": {"operator": round(t_op, 2), "preconditioner": round(t_pc, 2)},
+        "setup_s": {"operator": round(t_op, 2), "preconditioner": round(t_pc, 2), "inner_operator": round(t_in, 2)},
         "kernels_mixed": mixed["kernels"],
         "operator_levels": sizes,
         "roofline": {

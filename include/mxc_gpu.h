@@ -142,8 +142,12 @@ typedef struct {
   int max_cycles;      /* outer cycles */
   int mode;            /* MXG_GMRES_* */
   int inner_precision; /* MXG_FP64, or MXG_FP32 for FP32 inner operator applications */
+  mxg_h2_t inner_operator; /* NULL: A; else the H^2 matrix applied in the inner (Arnoldi)
+                              iterations, e.g. A built at a looser tolerance, while the
+                              residual of every cycle uses A (iterative refinement) */
 } mxg_gmres_opts_t;
-/* Solves A x = b with left preconditioner M. x holds the initial guess on entry.
+/* Solves A x = b with left preconditioner M. x holds the initial guess on entry
+ * (a zero guess skips the first operator application: r = b).
  * resid[0..max_cycles]: relative residual (H^2 operator) after each cycle.
  * iters_out: cycles run; inner_out: total inner iterations. */
 int mxg_gmres(mxg_h2_t A, mxg_ulv_t M, const mxg_gmres_opts_t* opts, const double* d_b, double* d_x,

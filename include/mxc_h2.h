@@ -81,6 +81,8 @@ int mxc_solver_matvec(mxc_solver_t s, const double* d_x, double* d_y, int precis
 /* info: levels + 1 ints (singular boxes per level) */
 int mxc_solver_factorize(mxc_solver_t s, int precision, int* info);
 int mxc_solver_precondition(mxc_solver_t s, double* d_x);
+/* o->inner_operator may name the device matrix of another solver
+ * (mxc_solver_device) used for the inner iterations. */
 int mxc_solver_gmres(mxc_solver_t A, mxc_solver_t M, const mxg_gmres_opts_t* o, const double* d_b, double* d_x,
                      double* resid, long long* iters, long long* inner);
 /* dense FP64 check y = K x over the problem's bodies */

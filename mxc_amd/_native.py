@@ -29,7 +29,8 @@ class MxgKernel(C.Structure):
 
 
 class MxgGmresOpts(C.Structure):
-    _fields_ = [("tol", c_d), ("restart", c_int), ("max_cycles", c_int), ("mode", c_int), ("inner_precision", c_int)]
+    _fields_ = [("tol", c_d), ("restart", c_int), ("max_cycles", c_int), ("mode", c_int), ("inner_precision", c_int),
+                ("inner_operator", C.c_void_p)]
 
 
 class MxcH2Opts(C.Structure):

@@ -129,8 +129,8 @@ struct Gmres {
     if (mxg_ulv_solve(M, x, 1, n) != MXG_OK) throw Error(MXG_ERR_STATE, std::string("preconditioner: ") + mxg_last_error());
   }
 
-  void matvec(const double* x, double* y, int precision) {
-    if (mxg_h2_matvec(A, x, y, 1, n, precision) != MXG_OK)
+  void matvec(const double* x, double* y, int precision, mxg_h2_s* op = nullptr) {
+    if (mxg_h2_matvec(op ? op : A, x, y, 1, n, precision) != MXG_OK)
       throw Error(MXG_ERR_STATE, std::string("matvec: ") + mxg_last_error());
   }
 };
@@ -185,6 +185,8 @@ extern "C" int mxg_gmres(mxg_h2_t A, mxg_ulv_t M, const mxg_gmres_opts_t* o, con
     MXG_REQUIRE(o->restart >= 1 && o->restart <= 1000 && o->max_cycles >= 0, "bad restart/max_cycles");
     MXG_REQUIRE(o->mode == MXG_GMRES_REFERENCE || o->mode == MXG_GMRES_CGS2, "unknown GMRES mode");
     MXG_REQUIRE(o->inner_precision == MXG_FP64 || o->inner_precision == MXG_FP32, "unknown inner precision");
+    mxg_h2_s* Ain = o->inner_operator;
+    MXG_REQUIRE(!Ain || (Ain->n == A->n && Ain->ctx == A->ctx), "inner operator must act on the same bodies and device");
     MXG_HIP(hipSetDevice(A->ctx->device));
     Gmres g(A, M, o->restart);
     const long long n = g.n;
@@ -192,10 +194,14 @@ extern "C" int mxg_gmres(mxg_h2_t A, mxg_ulv_t M, const mxg_gmres_opts_t* o, con
     const bool ref = (o->mode == MXG_GMRES_REFERENCE);
     double normb = g.norm(d_b);
     if (normb == 0.) normb = 1.;
-    // r = b - A x
-    g.matvec(d_x, g.r.p, MXG_FP64);
-    hipLaunchKernelGGL(k_b_minus, dim3(g.grid()), dim3(256), 0, g.s, d_b, n, g.r.p);
-    MXG_HIP(hipGetLastError());
+    // r = b - A x (r = b for a zero initial guess)
+    if (g.norm(d_x) == 0.) {
+      MXG_HIP(hipMemcpyAsync(g.r.p, d_b, n * sizeof(double), hipMemcpyDeviceToDevice, g.s));
+    } else {
+      g.matvec(d_x, g.r.p, MXG_FP64);
+      hipLaunchKernelGGL(k_b_minus, dim3(g.grid()), dim3(256), 0, g.s, d_b, n, g.r.p);
+      MXG_HIP(hipGetLastError());
+    }
     resid[0] = g.norm(g.r.p) / normb;
     long long iters = 0, inner_total = 0;
     std::vector<double> H((size_t)(m + 1) * m), h(m + 1), h2(m + 1), gv(m + 1), cs(m), sn(m), y;
@@ -214,7 +220,7 @@ extern "C" int mxg_gmres(mxg_h2_t A, mxg_ulv_t M, const mxg_gmres_opts_t* o, con
       int k = 0;
       for (int i = 0; i < m; i++) {
         double* vi = g.V.p + (size_t)i * n;
-        g.matvec(vi, g.w.p, o->inner_precision);
+        g.matvec(vi, g.w.p, o->inner_precision, Ain);
         g.precondition(g.w.p);
         g.dots(g.V.p, i + 1, g.w.p, h.data());
         g.sub_vh(i + 1, h.data());

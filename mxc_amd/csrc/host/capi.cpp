@@ -260,7 +260,7 @@ int mxc_solver_gmres(mxc_solver_t A, mxc_solver_t M, const mxg_gmres_opts_t* o, 
                      double* resid, long long* iters, long long* inner) {
   return guard([&] {
     require(A && M && o && resid, "bad arguments");
-    A->s.solveGMRES(o->tol, M->s, d_x, d_b, o->restart, o->max_cycles, o->mode, o->inner_precision);
+    A->s.solveGMRES(o->tol, M->s, d_x, d_b, o->restart, o->max_cycles, o->mode, o->inner_precision, o->inner_operator);
     std::copy(A->s.resid.begin(), A->s.resid.begin() + (A->s.iters + 1), resid);
     if (iters) *iters = A->s.iters;
     if (inner) *inner = A->s.inner_iters;

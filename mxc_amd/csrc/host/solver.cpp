@@ -119,9 +119,9 @@ void H2MatrixSolver::solvePrecondition(double* X_dev) {
 }
 
 void H2MatrixSolver::solveGMRES(double tol, H2MatrixSolver& M, double* X_dev, const double* B_dev, long long inner,
-                                long long outer, int mode, int inner_precision) {
+                                long long outer, int mode, int inner_precision, mxg_h2_t inner_operator) {
   if (!M.ulv) throw std::runtime_error("solveGMRES: preconditioner not factorized");
-  mxg_gmres_opts_t o{tol, (int)inner, (int)outer, mode, inner_precision};
+  mxg_gmres_opts_t o{tol, (int)inner, (int)outer, mode, inner_precision, inner_operator};
   resid.assign(outer + 1, 0.);
   check(mxg_gmres(dev, M.ulv, &o, B_dev, X_dev, resid.data(), &iters, &inner_iters));
 }

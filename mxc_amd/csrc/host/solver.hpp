@@ -54,8 +54,12 @@ class H2MatrixSolver {
   void solvePrecondition(double* X_dev);
   // GMRES (solver.cpp:144-201) with this matrix as the operator and M as the
   // left preconditioner. X holds the initial guess.
+  // inner_operator (optional): the device matrix applied in the inner
+  // iterations (e.g. the `dev` of this operator built at a looser tolerance);
+  // the residual of every cycle uses this matrix.
   void solveGMRES(double tol, H2MatrixSolver& M, double* X_dev, const double* B_dev, long long inner_iters_per_cycle,
-                  long long outer_iters, int mode = MXG_GMRES_REFERENCE, int inner_precision = MXG_FP64);
+                  long long outer_iters, int mode = MXG_GMRES_REFERENCE, int inner_precision = MXG_FP64,
+                  mxg_h2_t inner_operator = nullptr);
   // solver.cpp:229-238 (host arrays).
   static double solveRelErr(long long lenX, const double X[], const double ref[]);
 

@@ -162,8 +162,12 @@ class H2:
         N.check_h2(N.h2().mxc_solver_precondition(self.handle, x.ptr))
 
     def gmres(self, M: "H2", b: DeviceArray, x: DeviceArray, tol: float, restart: int = 10, max_cycles: int = 50,
-              mode: int = N.GMRES_REFERENCE, inner_precision: int = N.MXG_FP64):
-        o = N.MxgGmresOpts(tol, restart, max_cycles, mode, inner_precision)
+              mode: int = N.GMRES_REFERENCE, inner_precision: int = N.MXG_FP64, inner: "H2" = None):
+        """GMRES on this operator with left preconditioner M. `inner` (optional)
+        is the operator applied in the inner iterations; every cycle's residual
+        uses this one."""
+        inner_dev = N.h2().mxc_solver_device(inner.handle) if inner is not None else None
+        o = N.MxgGmresOpts(tol, restart, max_cycles, mode, inner_precision, inner_dev)
         resid = np.zeros(max_cycles + 1)
         it = C.c_longlong()
         inner = C.c_longlong()

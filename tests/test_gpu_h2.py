@@ -198,3 +198,26 @@ def test_gmres_cgs2_reaches_tolerance(setup, operators, preconditioners, inner_p
     dx2 = ctx.zeros(NPTS)
     A.gmres(M, db, dx2, 1e-11, restart=10, max_cycles=50, mode=N.GMRES_REFERENCE)
     assert O.rel_err(dx.download(), dx2.download()) < 1e-9
+
+
+def test_gmres_with_looser_inner_operator(setup, operators, preconditioners):
+    """Iterative refinement: FP32 inner iterations on an operator built at
+    1e-6, every cycle's residual on the accurate operator, reaches the same
+    solution as the plain FP64 solve."""
+    A, _ = operators
+    M, _ = preconditioners
+    ctx = setup["ctx"]
+    A_in = H2.operator(setup["prob"], 1e-6, SAMP)
+    b = O.uniform_vec(NPTS, -1, 1, 13)
+    db, dx = ctx.array(b), ctx.zeros(NPTS)
+    resid, iters, inner = A.gmres(M, db, dx, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2,
+                                  inner_precision=N.MXG_FP32, inner=A_in)
+    assert resid[-1] < 1e-11, resid
+    assert 1 <= iters <= 8
+    dx2 = ctx.zeros(NPTS)
+    r2, _, _ = A.gmres(M, db, dx2, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2)
+    assert O.rel_err(dx.download(), dx2.download()) < 1e-9
+    # a zero initial guess takes r = b; a nonzero one the operator residual
+    dx3 = ctx.array(dx.download())
+    r3, it3, _ = A.gmres(M, db, dx3, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2)
+    assert it3 == 0 and r3[0] < 1e-11
