@@ -8,8 +8,10 @@ leaf 256. The operator is an interpolative H^2 matrix at tolerance OP_EPS
 matrix truncated at 1e-6 with rank cap 64, ULV-factorized. A step is one
 solve: GMRES from b (U(-1, 1), a fresh seed per step) to
 ||b - A x|| / ||b|| <= 1e-10 on the FP64 H^2 operator, restart 30, with the
-factorization resident in HBM. The mixed path runs the ULV in FP32 and the
-inner operator applications in FP32 (FP64 Arnoldi, Givens, outer residual);
+factorization resident in HBM. The mixed path factorizes the ULV in FP32,
+stores the factors in bfloat16 for the substitution (FP32 accumulation) and
+runs the inner operator applications in FP32 (FP64 Arnoldi, Givens, outer
+residual);
 the FP64 path runs everything in FP64. The mixed path's inner iterations
 apply the operator built at the looser tolerance INNER_EPS (GMRES iterative
 refinement: every cycle's residual uses the FP64 operator at OP_EPS).
@@ -43,6 +45,8 @@ def parse():
     ap.add_argument("--sampling", type=int, default=2000)
     ap.add_argument("--inner-eps", type=float, default=1e-7,
                     help="tolerance of the operator the mixed path applies in its FP32 inner iterations (0: the main operator)")
+    ap.add_argument("--mixed-factor", choices=["bf16", "fp32"], default="bf16",
+                    help="ULV of the mixed path: FP32 factorization with bfloat16-stored or FP32-stored factors")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=16384, help="points of the CPU baseline sample")
     return ap.parse_args()
@@ -147,7 +151,7 @@ def main():
                     kernels=kern)
 
     fp64 = run_path(N.MXG_FP64, N.MXG_FP64, 0)
-    mixed = run_path(N.MXG_FP32, N.MXG_FP32, 1)
+    mixed = run_path(N.MXG_BF16 if args.mixed_factor == "bf16" else N.MXG_FP32, N.MXG_FP32, 1)
 
     # roofline of the dominant kernel of the mixed path: the FP32 product
     # launch of the matvec (every leaf near block and every coupling block of
@@ -181,7 +185,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 ULV + f32 inner operator, f64 Arnoldi/residual",
+        "dtype": ("f32 ULV factorization with bf16-stored factors" if args.mixed_factor == "bf16" else "f32 ULV") +
+                 " + f32 inner operator, f64 Arnoldi/residual",
         "data": "synthetic (seeded points and right-hand sides)",
         "config": {
             "workload": "H2 GMRES solve, unit-sphere Yukawa N=2^20, leaf 256",

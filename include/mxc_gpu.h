@@ -37,8 +37,10 @@ typedef struct {
   double alpha;  /* Yukawa decay / Gaussian width */
 } mxg_kernel_t;
 
-/* precision of a factorization or of an operator application */
-enum { MXG_FP64 = 0, MXG_FP32 = 1 };
+/* precision of a factorization or of an operator application; MXG_BF16 (ULV
+ * only): factorization in FP32, factors stored in bfloat16 for the
+ * substitution, which accumulates in FP32 */
+enum { MXG_FP64 = 0, MXG_FP32 = 1, MXG_BF16 = 2 };
 
 typedef struct mxg_ctx_s* mxg_ctx_t; /* device + stream (replaces deviceHandle_t) */
 typedef struct mxg_h2_s* mxg_h2_t;   /* one H^2 matrix resident on the device   */

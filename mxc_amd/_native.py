@@ -19,6 +19,7 @@ P = C.c_void_p
 
 MXG_FP64 = 0
 MXG_FP32 = 1
+MXG_BF16 = 2  # ULV only: FP32 factorization, bfloat16-stored factors for the substitution
 KERNELS = {"laplace": 0, "yukawa": 1, "gaussian": 2}
 GMRES_REFERENCE = 0
 GMRES_CGS2 = 1

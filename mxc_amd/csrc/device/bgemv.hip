@@ -7,31 +7,58 @@
 // 256-thread workgroup per block, so a level's work spreads over as many
 // workgroups as it has blocks, whatever its number of boxes. Within a block,
 // lanes run down the rows (16-byte vector loads of a column when the block
-// is aligned: 4 floats or 2 doubles per lane) and the four waves take every
-// fourth column; the waves' partial rows are summed through LDS in a fixed
-// order. The per-box sums then read the terms back in CSR order, so every
-// result is deterministic.
+// is aligned) and the four waves take every fourth column; short columns
+// share a wave. The partial rows are summed through LDS in a fixed order,
+// and the per-box sums read the terms back in CSR order, so every result is
+// deterministic.
+//
+// Blocks are stored in the compute type T (double or float) or, for the
+// FP32 substitution of a preconditioner, in bfloat16 (the upper 16 bits of
+// the FP32 pattern, rounded to nearest even); products always accumulate in T.
 #include "h2.hpp"
 
 namespace mxg {
 namespace {
 
-template <class T, int V> struct Vec;
+template <class T, int V> struct Vec { typedef T type __attribute__((ext_vector_type(V))); };
 template <> struct Vec<float, 1> { typedef float type; };
 template <> struct Vec<double, 1> { typedef double type; };
-template <> struct Vec<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
-template <> struct Vec<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
 
-// rows covered by one pass of a workgroup: 64 lanes x V rows x QN groups
-constexpr int QN = 4;
+// V elements of storage type S at p (16-byte aligned when V > 1) as T.
+template <class T, class S, int V>
+__device__ inline typename Vec<T, V>::type load_vec(const S* p) {
+  if constexpr (std::is_same<T, S>::value) {
+    return *reinterpret_cast<const typename Vec<T, V>::type*>(p);
+  } else {
+    static_assert(std::is_same<T, float>::value && std::is_same<S, bf16_t>::value, "bf16 storage needs float compute");
+    typename Vec<T, V>::type out;
+    if constexpr (V == 1) {
+      out = __uint_as_float((unsigned)p[0] << 16);
+    } else {
+      typedef unsigned short U __attribute__((ext_vector_type(V)));
+      const U u = *reinterpret_cast<const U*>(p);
+#pragma unroll
+      for (int e = 0; e < V; e++) out[e] = __uint_as_float((unsigned)u[e] << 16);
+    }
+    return out;
+  }
+}
+
+// row groups of one pass of a workgroup (64 lanes x V rows x QN groups):
+// 256 rows for V <= 2, 512 rows for V >= 4
+template <int V>
+constexpr int qn() {
+  return V == 1 ? 4 : V == 2 ? 2 : V == 4 ? 2 : 1;
+}
 
 // Partial rows of B[r0 : r0 + 64 V QN, :] v for this wave's columns (c = wave,
 // wave + 4, ...) into red[wave][...], then the sum over the waves for rows
 // r < nrows is handed to emit(r, value) by thread r (and r + 256, ...).
-template <class T, int V, class VF, class EF>
-__device__ inline void wg_gemv_pass(const T* __restrict__ B, long long ld, int n, int m, int r0, VF vfun, EF emit,
+template <class T, class S, int V, class VF, class EF>
+__device__ inline void wg_gemv_pass(const S* __restrict__ B, long long ld, int n, int m, int r0, VF vfun, EF emit,
                                     T* red) {
   typedef typename Vec<T, V>::type VT;
+  constexpr int QN = qn<V>();
   constexpr int ROWS = 64 * V * QN;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   VT acc[QN];
@@ -45,11 +72,11 @@ __device__ inline void wg_gemv_pass(const T* __restrict__ B, long long ld, int n
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       vv[u] = vfun(c + 4 * u);
-      const T* col = B + (long long)(c + 4 * u) * ld + r0;
+      const S* col = B + (long long)(c + 4 * u) * ld + r0;
 #pragma unroll
       for (int q = 0; q < QN; q++) {
         const int r = (q * 64 + lane) * V;
-        a[u][q] = (r < nrows) ? *reinterpret_cast<const VT*>(col + r) : VT(0);
+        a[u][q] = (r < nrows) ? load_vec<T, S, V>(col + r) : VT(0);
       }
     }
 #pragma unroll
@@ -57,11 +84,11 @@ __device__ inline void wg_gemv_pass(const T* __restrict__ B, long long ld, int n
   }
   for (; c < m; c += 4) {
     const T v0 = vfun(c);
-    const T* col0 = B + (long long)c * ld + r0;
+    const S* col0 = B + (long long)c * ld + r0;
 #pragma unroll
     for (int q = 0; q < QN; q++) {
       const int r = (q * 64 + lane) * V;
-      if (r < nrows) acc[q] += *reinterpret_cast<const VT*>(col0 + r) * v0;
+      if (r < nrows) acc[q] += load_vec<T, S, V>(col0 + r) * v0;
     }
   }
 #pragma unroll
@@ -77,8 +104,8 @@ __device__ inline void wg_gemv_pass(const T* __restrict__ B, long long ld, int n
 // Short columns (n <= 64 V): a column takes lpc = 2^ceil(log2(n / V)) lanes
 // and each wave works on 64 / lpc columns at once, so no lane idles for want
 // of rows. The 4 * 64 / lpc partial columns are summed through LDS.
-template <class T, int V, class VF, class EF>
-__device__ inline void wg_gemv_narrow(const T* __restrict__ B, long long ld, int n, int m, VF vfun, EF emit, T* red) {
+template <class T, class S, int V, class VF, class EF>
+__device__ inline void wg_gemv_narrow(const S* __restrict__ B, long long ld, int n, int m, VF vfun, EF emit, T* red) {
   typedef typename Vec<T, V>::type VT;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nv = (n + V - 1) / V;
@@ -96,15 +123,14 @@ __device__ inline void wg_gemv_narrow(const T* __restrict__ B, long long ld, int
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       vv[u] = vfun(c + u * stride);
-      a[u] = active ? *reinterpret_cast<const VT*>(B + (long long)(c + u * stride) * ld + r) : VT(0);
+      a[u] = active ? load_vec<T, S, V>(B + (long long)(c + u * stride) * ld + r) : VT(0);
     }
     acc += (a[0] * vv[0] + a[1] * vv[1]) + (a[2] * vv[2] + a[3] * vv[3]);
   }
   for (; c < m; c += stride)
-    if (active) acc += *reinterpret_cast<const VT*>(B + (long long)c * ld + r) * vfun(c);
+    if (active) acc += load_vec<T, S, V>(B + (long long)c * ld + r) * vfun(c);
   const int W = lpc * V;  // row stride of the partial columns in LDS
-  if (active) *reinterpret_cast<VT*>(red + (wave * G + g) * W + r) = acc;
-  else if (r < W) *reinterpret_cast<VT*>(red + (wave * G + g) * W + r) = VT(0);
+  *reinterpret_cast<VT*>(red + (wave * G + g) * W + r) = active ? acc : VT(0);
   __syncthreads();
   const int slots = 4 * G;
   for (int rr = threadIdx.x; rr < n; rr += 256) {
@@ -115,21 +141,32 @@ __device__ inline void wg_gemv_narrow(const T* __restrict__ B, long long ld, int
   __syncthreads();
 }
 
-template <class T, int V, class VF, class EF>
-__device__ inline void wg_gemv(const T* __restrict__ B, long long ld, int n, int m, VF vfun, EF emit, T* red) {
-  if (n <= 64 * V) {
-    wg_gemv_narrow<T, V>(B, ld, n, m, vfun, emit, red);
-    return;
-  }
-  for (int r0 = 0; r0 < n; r0 += 64 * V * QN) wg_gemv_pass<T, V>(B, ld, n, m, r0, vfun, emit, red);
+// LDS elements the block product needs with V rows per lane
+template <int V>
+constexpr int red_elems() {
+  return 4 * 64 * V * qn<V>();
 }
 
-template <class T>
+template <class T, class S, int V, class VF, class EF>
+__device__ inline void wg_gemv(const S* __restrict__ B, long long ld, int n, int m, VF vfun, EF emit, T* red) {
+  if (n <= 64 * V) {
+    wg_gemv_narrow<T, S, V>(B, ld, n, m, vfun, emit, red);
+    return;
+  }
+  for (int r0 = 0; r0 < n; r0 += 64 * V * qn<V>()) wg_gemv_pass<T, S, V>(B, ld, n, m, r0, vfun, emit, red);
+}
+
+template <class S>
+constexpr int vwidth() {
+  return 16 / (int)sizeof(S);
+}
+
+template <class T, class S>
 __global__ void __launch_bounds__(256) k_pair_gemv(const PairJob* __restrict__ jobs) {
-  constexpr int VW = 16 / (int)sizeof(T);
-  __shared__ __align__(16) T red[4 * 64 * VW * QN];
+  constexpr int VW = vwidth<S>();
+  __shared__ __align__(16) T red[red_elems<VW>()];
   const PairJob t = jobs[blockIdx.x];
-  const T* B = static_cast<const T*>(t.B);
+  const S* B = static_cast<const S*>(t.B);
   const T* vb = static_cast<const T*>(t.v);
   T* ob = static_cast<T*>(t.out);
   if (t.m == 0) {
@@ -139,9 +176,9 @@ __global__ void __launch_bounds__(256) k_pair_gemv(const PairJob* __restrict__ j
   auto vf = [&](int c) { return vb[c]; };
   auto ef = [&](int r, T s) { ob[r] = s; };
   if (t.vec)
-    wg_gemv<T, VW>(B, t.ld, t.n, t.m, vf, ef, red);
+    wg_gemv<T, S, VW>(B, t.ld, t.n, t.m, vf, ef, red);
   else
-    wg_gemv<T, 1>(B, t.ld, t.n, t.m, vf, ef, red);
+    wg_gemv<T, S, 1>(B, t.ld, t.n, t.m, vf, ef, red);
 }
 
 // Downward pass of one level: w_i = w_in_i + sum_far t_p (k_i entries), then
@@ -154,7 +191,7 @@ __global__ void __launch_bounds__(256) k_comb_down(const int* __restrict__ dims,
                                                    const long long* __restrict__ toffN, const T* __restrict__ Q,
                                                    const T* __restrict__ w_in, const T* __restrict__ tF,
                                                    const T* __restrict__ tN, T* __restrict__ y) {
-  __shared__ __align__(16) T red[4 * 64 * V * QN];
+  __shared__ __align__(16) T red[red_elems<V>()];
   extern __shared__ __align__(16) unsigned char dsm[];
   T* w = reinterpret_cast<T*>(dsm);
   const int i = blockIdx.x;
@@ -175,7 +212,7 @@ __global__ void __launch_bounds__(256) k_comb_down(const int* __restrict__ dims,
     for (int r = threadIdx.x; r < M; r += 256) emit(r, T(0));
     return;
   }
-  wg_gemv<T, V>(Q + offQ[i], M, M, k, [&](int c) { return w[c]; }, emit, red);
+  wg_gemv<T, T, V>(Q + offQ[i], M, M, k, [&](int c) { return w[c]; }, emit, red);
 }
 
 // Forward substitution, skeleton part: z_i = x_i[:s] - sum_near t_p.
@@ -194,14 +231,14 @@ __global__ void __launch_bounds__(256) k_comb_fwd(const int* __restrict__ ranks,
 }
 
 // Backward substitution: v_i = [w_i ; x_i[s:] - sum_near t_p], y_i = U_i v_i.
-template <class T, int V>
+template <class T, class S, int V>
 __global__ void __launch_bounds__(256) k_comb_bwd(const int* __restrict__ dims, const int* __restrict__ ranks,
                                                   const long long* __restrict__ offQ, const long long* __restrict__ offX,
                                                   const long long* __restrict__ offZ, const long long* __restrict__ nrp,
-                                                  const long long* __restrict__ toff, const T* __restrict__ U,
+                                                  const long long* __restrict__ toff, const S* __restrict__ U,
                                                   const T* __restrict__ x, const T* __restrict__ w,
                                                   const T* __restrict__ t, T* __restrict__ y) {
-  __shared__ __align__(16) T red[4 * 64 * V * QN];
+  __shared__ __align__(16) T red[red_elems<V>()];
   extern __shared__ __align__(16) unsigned char dsm[];
   T* xv = reinterpret_cast<T*>(dsm);
   const int i = blockIdx.x;
@@ -218,33 +255,37 @@ __global__ void __launch_bounds__(256) k_comb_bwd(const int* __restrict__ dims, 
   }
   __syncthreads();
   T* yi = y + offX[i];
-  wg_gemv<T, V>(U + offQ[i], M, M, M, [&](int c) { return xv[c]; }, [&](int r, T v) { yi[r] = v; }, red);
+  wg_gemv<T, S, V>(U + offQ[i], M, M, M, [&](int c) { return xv[c]; }, [&](int r, T v) { yi[r] = v; }, red);
 }
 
-template <class T>
-constexpr int vwidth() {
-  return 16 / (int)sizeof(T);
+__global__ void k_to_bf16(bf16_t* __restrict__ dst, const float* __restrict__ src, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const unsigned b = __float_as_uint(src[e]);
+    dst[e] = (bf16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+  }
 }
 
 }  // namespace
 
-template <class T>
-void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& tasks, const T* B, long long nB, const T* v,
+template <class T, class S>
+void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& tasks, const S* B, long long nB, const T* v,
                       long long nv, T* out, long long nout) {
-  constexpr long long W = vwidth<T>();
+  constexpr long long W = vwidth<S>();
   for (const PairTask& t : tasks) {
     MXG_REQUIRE(t.n >= 0 && t.m >= 0 && t.ld >= std::max(t.n, 1) && t.b >= 0 && t.v >= 0 && t.out >= 0, "bad pair task");
     MXG_REQUIRE(t.out + t.n <= nout && t.v + t.m <= nv, "pair task outside its vectors");
     MXG_REQUIRE(t.n == 0 || t.m == 0 || t.b + (long long)(t.m - 1) * t.ld + t.n <= nB, "pair task outside its matrix");
-    const T* b = B + t.b;
+    const S* b = B + t.b;
     const int vec = ((reinterpret_cast<uintptr_t>(b) % 16) == 0 && t.ld % W == 0 && t.n % W == 0) ? 1 : 0;
     jobs.push_back(PairJob{b, v + t.v, out + t.out, t.n, t.m, t.ld, vec});
   }
 }
-template void pair_jobs_append<float>(std::vector<PairJob>&, const std::vector<PairTask>&, const float*, long long,
-                                      const float*, long long, float*, long long);
-template void pair_jobs_append<double>(std::vector<PairJob>&, const std::vector<PairTask>&, const double*, long long,
-                                       const double*, long long, double*, long long);
+template void pair_jobs_append<float, float>(std::vector<PairJob>&, const std::vector<PairTask>&, const float*, long long,
+                                             const float*, long long, float*, long long);
+template void pair_jobs_append<double, double>(std::vector<PairJob>&, const std::vector<PairTask>&, const double*,
+                                               long long, const double*, long long, double*, long long);
+template void pair_jobs_append<float, bf16_t>(std::vector<PairJob>&, const std::vector<PairTask>&, const bf16_t*,
+                                              long long, const float*, long long, float*, long long);
 
 void pair_list_upload(PairList& pl, const std::vector<PairJob>& jobs, hipStream_t s) {
   MXG_REQUIRE(jobs.size() < (1ull << 31), "pair list too long");
@@ -252,14 +293,15 @@ void pair_list_upload(PairList& pl, const std::vector<PairJob>& jobs, hipStream_
   if (pl.count) pl.jobs.upload(jobs, s);
 }
 
-template <class T>
+template <class T, class S>
 void pair_gemv(const PairList& pl, hipStream_t s) {
   if (pl.count == 0) return;
-  hipLaunchKernelGGL(k_pair_gemv<T>, dim3((unsigned)pl.count), dim3(256), 0, s, pl.jobs.p);
+  hipLaunchKernelGGL((k_pair_gemv<T, S>), dim3((unsigned)pl.count), dim3(256), 0, s, pl.jobs.p);
   MXG_HIP(hipGetLastError());
 }
-template void pair_gemv<float>(const PairList&, hipStream_t);
-template void pair_gemv<double>(const PairList&, hipStream_t);
+template void pair_gemv<float, float>(const PairList&, hipStream_t);
+template void pair_gemv<double, double>(const PairList&, hipStream_t);
+template void pair_gemv<float, bf16_t>(const PairList&, hipStream_t);
 
 template <class T>
 void comb_down(const Level& Lv, bool vec, const long long* d_toffF, const long long* d_toffN, const T* Q, const T* w_in,
@@ -290,23 +332,32 @@ void comb_fwd(const Level& Lv, const long long* d_toff, const T* x, const T* t, 
 template void comb_fwd<float>(const Level&, const long long*, const float*, const float*, float*, hipStream_t);
 template void comb_fwd<double>(const Level&, const long long*, const double*, const double*, double*, hipStream_t);
 
-template <class T>
-void comb_bwd(const Level& Lv, bool vec, const long long* d_toff, const T* U, const T* x, const T* w, const T* t, T* y,
+template <class T, class S>
+void comb_bwd(const Level& Lv, bool vec, const long long* d_toff, const S* U, const T* x, const T* w, const T* t, T* y,
               hipStream_t s) {
   if (Lv.nb == 0) return;
   const size_t shm = std::max<long long>(Lv.maxM, 1) * sizeof(T);
   MXG_REQUIRE(shm <= 64 * 1024, "box too large for the backward pass");
   if (vec)
-    hipLaunchKernelGGL((k_comb_bwd<T, vwidth<T>()>), dim3((unsigned)Lv.nb), dim3(256), shm, s, Lv.d_dims.p, Lv.d_ranks.p,
+    hipLaunchKernelGGL((k_comb_bwd<T, S, vwidth<S>()>), dim3((unsigned)Lv.nb), dim3(256), shm, s, Lv.d_dims.p, Lv.d_ranks.p,
                        Lv.d_offQ.p, Lv.d_offX.p, Lv.d_offZ.p, Lv.d_nrp.p, d_toff, U, x, w, t, y);
   else
-    hipLaunchKernelGGL((k_comb_bwd<T, 1>), dim3((unsigned)Lv.nb), dim3(256), shm, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p,
-                       Lv.d_offX.p, Lv.d_offZ.p, Lv.d_nrp.p, d_toff, U, x, w, t, y);
+    hipLaunchKernelGGL((k_comb_bwd<T, S, 1>), dim3((unsigned)Lv.nb), dim3(256), shm, s, Lv.d_dims.p, Lv.d_ranks.p,
+                       Lv.d_offQ.p, Lv.d_offX.p, Lv.d_offZ.p, Lv.d_nrp.p, d_toff, U, x, w, t, y);
   MXG_HIP(hipGetLastError());
 }
-template void comb_bwd<float>(const Level&, bool, const long long*, const float*, const float*, const float*,
-                              const float*, float*, hipStream_t);
-template void comb_bwd<double>(const Level&, bool, const long long*, const double*, const double*, const double*,
-                               const double*, double*, hipStream_t);
+template void comb_bwd<float, float>(const Level&, bool, const long long*, const float*, const float*, const float*,
+                                     const float*, float*, hipStream_t);
+template void comb_bwd<double, double>(const Level&, bool, const long long*, const double*, const double*, const double*,
+                                       const double*, double*, hipStream_t);
+template void comb_bwd<float, bf16_t>(const Level&, bool, const long long*, const bf16_t*, const float*, const float*,
+                                      const float*, float*, hipStream_t);
+
+void to_bf16(bf16_t* dst, const float* src, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks), dim3(256), 0, s, dst, src, n);
+  MXG_HIP(hipGetLastError());
+}
 
 }  // namespace mxg

@@ -524,14 +524,14 @@ void ensure_matvec_jobs(mxg_h2_s* h) {
     Level& Lv = h->lv[l];
     std::vector<PairJob> up;
     const T* xin = (l == L) ? Ops<T>::x(h) : Ops<T>::Z(h, l + 1);
-    pair_jobs_append<T>(up, Lv.up_t, Ops<T>::QT(Lv), (long long)Lv.offQT[Lv.nb], xin, Lv.lenX, Ops<T>::Z(h, l),
+    pair_jobs_append<T, T>(up, Lv.up_t, Ops<T>::QT(Lv), (long long)Lv.offQT[Lv.nb], xin, Lv.lenX, Ops<T>::Z(h, l),
                         std::max<long long>(Lv.lenZ, 1));
     pair_list_upload(Lv.up_pl[pi], up, s);
-    pair_jobs_append<T>(prod, Lv.far_t, Ops<T>::C(Lv), (long long)Lv.C.n, Ops<T>::Z(h, l), std::max<long long>(Lv.lenZ, 1),
+    pair_jobs_append<T, T>(prod, Lv.far_t, Ops<T>::C(Lv), (long long)Lv.C.n, Ops<T>::Z(h, l), std::max<long long>(Lv.lenZ, 1),
                         tb.p + Lv.tbaseF, Lv.lenTF);
   }
   Level& Lf = h->lv[L];
-  pair_jobs_append<T>(prod, Lf.near_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, tb.p + Lf.tbaseN, Lf.lenTN);
+  pair_jobs_append<T, T>(prod, Lf.near_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, tb.p + Lf.tbaseN, Lf.lenTN);
   pair_list_upload(h->prod_pl[pi], prod, s);
   MXG_HIP(hipStreamSynchronize(s));
   h->jobs_ready[pi] = true;
@@ -555,12 +555,12 @@ void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y) {
   for (long long l = L; l >= 0; l--) {
     if (h->lv[l].lenZ == 0) continue;
     ProfScope prof(h->ctx, MXG_PROF_MATVEC_UPPER);
-    pair_gemv<T>(h->lv[l].up_pl[pi], s);
+    pair_gemv<T, T>(h->lv[l].up_pl[pi], s);
   }
   // every far term of every level and the near terms of the leaf, one launch
   {
     ProfScope prof(h->ctx, f64 ? MXG_PROF_MATVEC_LEAF_FP64 : MXG_PROF_MATVEC_LEAF_FP32);
-    pair_gemv<T>(h->prod_pl[pi], s);
+    pair_gemv<T, T>(h->prod_pl[pi], s);
   }
   // downward pass: w = w_in + far terms, y = Q w (+ near terms at the leaf)
   for (long long l = 0; l <= L; l++) {

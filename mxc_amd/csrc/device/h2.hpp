@@ -97,15 +97,20 @@ template <class T>
 void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y);
 
 // ---- block-sparse products (bgemv.hip) ----
-// Appends the jobs of tasks over the arrays B (nB elements), v (nv) and out
-// (nout), after checking every task against those sizes.
-template <class T>
-void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& tasks, const T* B, long long nB, const T* v,
+// bfloat16 storage: the upper half of the FP32 bit pattern
+typedef unsigned short bf16_t;
+// Appends the jobs of tasks over the arrays B (nB elements, stored as S), v
+// (nv) and out (nout), after checking every task against those sizes.
+template <class T, class S>
+void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& tasks, const S* B, long long nB, const T* v,
                       long long nv, T* out, long long nout);
 void pair_list_upload(PairList& pl, const std::vector<PairJob>& jobs, hipStream_t s);
-// out[r] = sum_c B[r + c*ld] v[c] for every job, one workgroup per job
-template <class T>
+// out[r] = sum_c B[r + c*ld] v[c] for every job, one workgroup per job,
+// blocks stored as S, arithmetic in T
+template <class T, class S>
 void pair_gemv(const PairList& pl, hipStream_t s);
+// FP32 -> bfloat16, round to nearest even
+void to_bf16(bf16_t* dst, const float* src, size_t n, hipStream_t s);
 // w_i = w_in_i + sum_far tF, y_i = Q_i[:, :k] w_i (+ sum_near tN when tN != nullptr)
 template <class T>
 void comb_down(const Level& Lv, bool vec, const long long* d_toffF, const long long* d_toffN, const T* Q, const T* w_in,
@@ -113,8 +118,8 @@ void comb_down(const Level& Lv, bool vec, const long long* d_toffF, const long l
 // z_i = x_i[:s] - sum_near t
 template <class T>
 void comb_fwd(const Level& Lv, const long long* d_toff, const T* x, const T* t, T* z, hipStream_t s);
-// y_i = U_i [w_i ; x_i[s:] - sum_near t]
-template <class T>
-void comb_bwd(const Level& Lv, bool vec, const long long* d_toff, const T* U, const T* x, const T* w, const T* t, T* y,
+// y_i = U_i [w_i ; x_i[s:] - sum_near t], U stored as S
+template <class T, class S>
+void comb_bwd(const Level& Lv, bool vec, const long long* d_toff, const S* U, const T* x, const T* w, const T* t, T* y,
               hipStream_t s);
 }  // namespace mxg

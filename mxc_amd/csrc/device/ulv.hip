@@ -56,11 +56,15 @@ struct UlvBase {
   virtual void info(int* out) = 0;
 };
 
-template <class T>
+// T: factorization and substitution arithmetic; S: storage of the factors the
+// substitution streams (T, or bfloat16 with T = float).
+template <class T, class S = T>
 struct UlvImpl : UlvBase {
+  static constexpr bool kCopy = !std::is_same<T, S>::value;
   mxg_h2_s* h;
   struct Lv {
     DevBuf<T> A, U, V, RRinv, X, Z;
+    DevBuf<S> As, Us, Vs;  // the factors in storage precision (kCopy only)
     std::vector<long long> offRR;
     DevBuf<int> info;
     int info_sum = 0;
@@ -94,6 +98,11 @@ struct UlvImpl : UlvBase {
       F.X.alloc(std::max<long long>(H.lenX, 1));
       F.Z.alloc(std::max<long long>(H.lenZ, 1));
       F.info.alloc_zero(std::max<long long>(H.nb, 1), s);
+      if constexpr (kCopy) {
+        F.As.alloc(std::max<size_t>(F.A.n, 1));
+        F.Us.alloc(std::max<size_t>(F.U.n, 1));
+        F.Vs.alloc(std::max<size_t>(F.V.n, 1));
+      }
     }
     vin.alloc(h->n);
     long long tmax = 1;
@@ -112,6 +121,16 @@ struct UlvImpl : UlvBase {
     MXG_HIP(hipStreamSynchronize(s));
   }
 
+  static const S* Astore(Lv& F) {
+    if constexpr (kCopy) return F.As.p; else return F.A.p;
+  }
+  static const S* Ustore(Lv& F) {
+    if constexpr (kCopy) return F.Us.p; else return F.U.p;
+  }
+  static const S* Vstore(Lv& F) {
+    if constexpr (kCopy) return F.Vs.p; else return F.V.p;
+  }
+
   // Jobs of the substitution at level l (terms in tbuf, tlen elements).
   void build_solve_lists(long long l, long long tlen) {
     hipStream_t s = h->ctx->stream;
@@ -123,7 +142,7 @@ struct UlvImpl : UlvBase {
     F.vecM = true;
     for (long long i = 0; i < nb; i++) {
       const long long M = H.dims[i];
-      if (M % (16 / (long long)sizeof(T))) F.vecM = false;
+      if (M % (16 / (long long)sizeof(S))) F.vecM = false;
       if (M > 0) vt.push_back(PairTask{H.offQ[i], H.offX[i], H.offX[i], (int)M, (int)M, (int)M, 0});
       const long long si = H.ranks[i];
       for (long long p = H.nrp[i]; p < H.nrp[i + 1]; p++) {
@@ -138,9 +157,9 @@ struct UlvImpl : UlvBase {
     const T* y = (l == h->L) ? vin.p : lv[l + 1].Z.p;
     const long long ylen = (l == h->L) ? h->n : std::max<long long>(h->lv[l + 1].lenZ, 1);
     std::vector<PairJob> jv, jf, jb;
-    pair_jobs_append<T>(jv, vt, F.V.p, H.offQ[nb], y, ylen, F.X.p, std::max<long long>(H.lenX, 1));
-    pair_jobs_append<T>(jf, ft, F.A.p, (long long)F.A.n, F.X.p, std::max<long long>(H.lenX, 1), tbuf.p, tlen);
-    pair_jobs_append<T>(jb, bt, F.A.p, (long long)F.A.n, F.Z.p, std::max<long long>(H.lenZ, 1), tbuf.p, tlen);
+    pair_jobs_append<T, S>(jv, vt, Vstore(F), H.offQ[nb], y, ylen, F.X.p, std::max<long long>(H.lenX, 1));
+    pair_jobs_append<T, S>(jf, ft, Astore(F), (long long)F.A.n, F.X.p, std::max<long long>(H.lenX, 1), tbuf.p, tlen);
+    pair_jobs_append<T, S>(jb, bt, Astore(F), (long long)F.A.n, F.Z.p, std::max<long long>(H.lenZ, 1), tbuf.p, tlen);
     pair_list_upload(F.v_pl, jv, s);
     pair_list_upload(F.fwd_pl, jf, s);
     pair_list_upload(F.bwd_pl, jb, s);
@@ -155,6 +174,14 @@ struct UlvImpl : UlvBase {
       if (l > 0) copy_next(l);
     }
     hipStream_t s = h->ctx->stream;
+    if constexpr (kCopy) {
+      for (long long l = 0; l <= h->L; l++) {
+        Lv& F = lv[l];
+        to_bf16(F.As.p, F.A.p, F.A.n, s);
+        to_bf16(F.Us.p, F.U.p, F.U.n, s);
+        to_bf16(F.Vs.p, F.V.p, F.V.n, s);
+      }
+    }
     for (long long l = 0; l <= h->L; l++) {
       std::vector<int> inf(h->lv[l].nb);
       MXG_HIP(hipMemcpyAsync(inf.data(), lv[l].info.p, inf.size() * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -389,9 +416,9 @@ struct UlvImpl : UlvBase {
       Level& H = h->lv[l];
       Lv& F = lv[l];
       ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_FWD_LEAF : -1);
-      pair_gemv<T>(F.v_pl, s);
+      pair_gemv<T, S>(F.v_pl, s);
       if (H.lenZ > 0) {
-        pair_gemv<T>(F.fwd_pl, s);
+        pair_gemv<T, S>(F.fwd_pl, s);
         comb_fwd<T>(H, F.d_toffU.p, F.X.p, tbuf.p, F.Z.p, s);
       }
     }
@@ -401,8 +428,8 @@ struct UlvImpl : UlvBase {
       // w of level l is the output of level l - 1 (stored in this level's Z buffer)
       T* out = (l == L) ? v : lv[l + 1].Z.p;
       ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_BWD_LEAF : -1);
-      pair_gemv<T>(F.bwd_pl, s);
-      comb_bwd<T>(H, F.vecM, F.d_toffB.p, F.U.p, F.X.p, F.Z.p, tbuf.p, out, s);
+      pair_gemv<T, S>(F.bwd_pl, s);
+      comb_bwd<T, S>(H, F.vecM, F.d_toffB.p, Ustore(F), F.X.p, F.Z.p, tbuf.p, out, s);
     }
     if constexpr (sizeof(T) == 8)
       MXG_HIP(hipMemcpyAsync(x, v, h->n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -433,7 +460,7 @@ int mxg_ulv_create(mxg_h2_t h, int precision, mxg_ulv_t* out) {
     MXG_REQUIRE(h && out, "null argument");
     for (long long l = 0; l <= h->L; l++) MXG_REQUIRE(h->lv[l].built, "H2 matrix not fully constructed");
     MXG_REQUIRE(h->flags & MXG_H2_FACTORABLE, "H2 matrix was created without MXG_H2_FACTORABLE");
-    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32, "unknown precision");
+    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32 || precision == MXG_BF16, "unknown precision");
     MXG_HIP(hipSetDevice(h->ctx->device));
     auto f = std::make_unique<mxg_ulv_s>();
     f->h = h;
@@ -441,8 +468,10 @@ int mxg_ulv_create(mxg_h2_t h, int precision, mxg_ulv_t* out) {
     auto t0 = std::chrono::steady_clock::now();
     if (precision == MXG_FP64)
       f->impl = std::make_unique<UlvImpl<double>>(h);
-    else
+    else if (precision == MXG_FP32)
       f->impl = std::make_unique<UlvImpl<float>>(h);
+    else
+      f->impl = std::make_unique<UlvImpl<float, bf16_t>>(h);
     f->impl->factorize();
     MXG_HIP(hipStreamSynchronize(h->ctx->stream));
     f->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -144,6 +144,15 @@ def test_factorization_substitution_matches_oracle(setup, preconditioners):
     z32 = d.download()
     assert O.rel_err(z32, z) < 2 * serr, (O.rel_err(z32, z), serr)
     assert O.rel_err(z32, setup["x"]) < 2 * serr
+    # FP32 factorization with bfloat16-stored factors: the substitution
+    # differs from the FP32 one by bfloat16 rounding (2^-9 relative per
+    # entry) amplified by the conditioning of the rotated blocks
+    M.factorize(N.MXG_BF16)
+    d = ctx.array(setup["bref"])
+    M.precondition(d)
+    z16 = d.download()
+    assert 1e-4 < O.rel_err(z16, z32) < 0.1, O.rel_err(z16, z32)
+    assert O.rel_err(z16, setup["x"]) < 0.1
     M.factorize(N.MXG_FP64)
 
 
@@ -198,6 +207,27 @@ def test_gmres_cgs2_reaches_tolerance(setup, operators, preconditioners, inner_p
     dx2 = ctx.zeros(NPTS)
     A.gmres(M, db, dx2, 1e-11, restart=10, max_cycles=50, mode=N.GMRES_REFERENCE)
     assert O.rel_err(dx.download(), dx2.download()) < 1e-9
+
+
+def test_gmres_bf16_preconditioner(setup, operators, preconditioners):
+    """GMRES-IR with the bfloat16-stored ULV reaches the tolerance and the
+    FP64 solution."""
+    A, _ = operators
+    M, _ = preconditioners
+    ctx = setup["ctx"]
+    b = O.uniform_vec(NPTS, -1, 1, 17)
+    db = ctx.array(b)
+    dx64 = ctx.zeros(NPTS)
+    A.gmres(M, db, dx64, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2)
+    M.factorize(N.MXG_BF16)
+    try:
+        dx = ctx.zeros(NPTS)
+        resid, iters, inner = A.gmres(M, db, dx, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2,
+                                      inner_precision=N.MXG_FP32)
+        assert resid[-1] < 1e-11, resid
+        assert O.rel_err(dx.download(), dx64.download()) < 1e-9
+    finally:
+        M.factorize(N.MXG_FP64)
 
 
 def test_gmres_with_looser_inner_operator(setup, operators, preconditioners):
