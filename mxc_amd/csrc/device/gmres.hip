@@ -22,23 +22,40 @@
 namespace mxg {
 namespace {
 
-constexpr int DOT_ROWS = 8192;  // rows per block of the partial sums
+constexpr int DOT_ROWS = 2048;  // rows per block of the partial sums
+constexpr int DOT_MAXC = 32;    // columns per pass of k_dots
 
+// part[blk * ncols + c] = sum over the block's rows of V[:, c] * w
+// (w read once per row; columns in passes of DOT_MAXC)
 __global__ void __launch_bounds__(256) k_dots(const double* __restrict__ V, int ncols, long long ldv, long long n,
                                               const double* __restrict__ w, double* __restrict__ part) {
-  __shared__ double red[4];
+  __shared__ double red[4][DOT_MAXC];
   const long long r0 = (long long)blockIdx.x * DOT_ROWS;
   const long long r1 = min(r0 + DOT_ROWS, n);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int c = 0; c < ncols; c++) {
-    const double* v = V + (long long)c * ldv;
-    double s = 0.;
-    for (long long r = r0 + threadIdx.x; r < r1; r += 256) s += v[r] * w[r];
-    s = wave_sum(s);
+  for (int c0 = 0; c0 < ncols; c0 += DOT_MAXC) {
+    const int nc = min(DOT_MAXC, ncols - c0);
+    double s[DOT_MAXC];
+#pragma unroll
+    for (int c = 0; c < DOT_MAXC; c++) s[c] = 0.;
+    for (long long r = r0 + threadIdx.x; r < r1; r += 256) {
+      const double wr = w[r];
+#pragma unroll
+      for (int c = 0; c < DOT_MAXC; c++)
+        if (c < nc) s[c] += V[r + (long long)(c0 + c) * ldv] * wr;
+    }
+#pragma unroll
+    for (int c = 0; c < DOT_MAXC; c++) {
+      if (c < nc) {
+        const double t = wave_sum(s[c]);
+        if (lane == 0) red[wave][c] = t;
+      }
+    }
     __syncthreads();
-    if (lane == 0) red[wave] = s;
+    if ((int)threadIdx.x < nc)
+      part[(long long)blockIdx.x * ncols + c0 + threadIdx.x] =
+          (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
     __syncthreads();
-    if (threadIdx.x == 0) part[(long long)blockIdx.x * ncols + c] = (red[0] + red[1]) + (red[2] + red[3]);
   }
 }
 
@@ -48,6 +65,14 @@ __global__ void k_sum_parts(const double* __restrict__ part, int nblk, int ncols
   double s = 0.;
   for (int b = 0; b < nblk; b++) s += part[(long long)b * ncols + c];
   out[c] = s;
+}
+
+// y = x / sqrt(nrm2[0]) (zero when nrm2 is not positive)
+__global__ void k_scale_rsqrt(const double* __restrict__ x, const double* __restrict__ nrm2, long long n,
+                              double* __restrict__ y) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double v = nrm2[0];
+  if (r < n) y[r] = v > 0. ? x[r] / sqrt(v) : 0.;
 }
 
 // w -= V h (h on the device, ncols entries)
@@ -90,6 +115,7 @@ struct Gmres {
   hipStream_t s;
   int nblk;
   DevBuf<double> V, w, r, part, hd, yd;
+  double* hpin = nullptr;  // pinned host copy of one Arnoldi step: h, h2, |w|^2
 
   Gmres(mxg_h2_s* a, mxg_ulv_t mm, int restart) : A(a), M(mm), n(a->n), m(restart), s(a->ctx->stream) {
     nblk = (int)ceil_div(n, DOT_ROWS);
@@ -97,42 +123,42 @@ struct Gmres {
     w.alloc(n);
     r.alloc(n);
     part.alloc((size_t)nblk * (m + 1));
-    hd.alloc(m + 1);
+    hd.alloc(2 * (m + 1) + 1);
     yd.alloc(m + 1);
+    MXG_HIP(hipHostMalloc((void**)&hpin, (2 * (m + 1) + 1) * sizeof(double), hipHostMallocDefault));
   }
+  ~Gmres() {
+    if (hpin) (void)hipHostFree(hpin);
+  }
+  Gmres(const Gmres&) = delete;
+  Gmres& operator=(const Gmres&) = delete;
 
   unsigned grid() const { return ceil_div(n, 256); }
 
-  // h[0..ncols) = V[:, :ncols]^T w, copied to the host
-  void dots(const double* Vp, int ncols, const double* wp, double* h_host) {
+  // out[0..ncols) = V[:, :ncols]^T w on the device
+  void dots_dev(const double* Vp, int ncols, const double* wp, double* out) {
     hipLaunchKernelGGL(k_dots, dim3(nblk), dim3(256), 0, s, Vp, ncols, n, n, wp, part.p);
     MXG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64 * ((ncols + 63) / 64)), 0, s, part.p, nblk, ncols, hd.p);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64 * ((ncols + 63) / 64)), 0, s, part.p, nblk, ncols, out);
     MXG_HIP(hipGetLastError());
-    MXG_HIP(hipMemcpyAsync(h_host, hd.p, ncols * sizeof(double), hipMemcpyDeviceToHost, s));
-    MXG_HIP(hipStreamSynchronize(s));
   }
 
   double norm(const double* x) {
-    double v;
-    dots(x, 1, x, &v);
-    return std::sqrt(v);
+    dots_dev(x, 1, x, hd.p);
+    MXG_HIP(hipMemcpyAsync(hpin, hd.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    MXG_HIP(hipStreamSynchronize(s));
+    return std::sqrt(hpin[0]);
   }
 
-  void sub_vh(int ncols, const double* h_host) {
-    MXG_HIP(hipMemcpyAsync(hd.p, h_host, ncols * sizeof(double), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_sub_vh, dim3(grid()), dim3(256), 0, s, V.p, ncols, n, n, hd.p, w.p);
+  // w -= V[:, :ncols] h, h on the device
+  void sub_vh_dev(int ncols, const double* h) {
+    hipLaunchKernelGGL(k_sub_vh, dim3(grid()), dim3(256), 0, s, V.p, ncols, n, n, h, w.p);
     MXG_HIP(hipGetLastError());
   }
 
-  void precondition(double* x) {
-    if (mxg_ulv_solve(M, x, 1, n) != MXG_OK) throw Error(MXG_ERR_STATE, std::string("preconditioner: ") + mxg_last_error());
-  }
+  void precondition(double* x) { ulv_apply(M, x); }
 
-  void matvec(const double* x, double* y, int precision, mxg_h2_s* op = nullptr) {
-    if (mxg_h2_matvec(op ? op : A, x, y, 1, n, precision) != MXG_OK)
-      throw Error(MXG_ERR_STATE, std::string("matvec: ") + mxg_last_error());
-  }
+  void matvec(const double* x, double* y, int precision, mxg_h2_s* op = nullptr) { h2_apply(op ? op : A, x, y, precision); }
 };
 
 // Householder least squares min ||s - H y|| for H ((k+1) x k, ld), as the
@@ -204,7 +230,7 @@ extern "C" int mxg_gmres(mxg_h2_t A, mxg_ulv_t M, const mxg_gmres_opts_t* o, con
     }
     resid[0] = g.norm(g.r.p) / normb;
     long long iters = 0, inner_total = 0;
-    std::vector<double> H((size_t)(m + 1) * m), h(m + 1), h2(m + 1), gv(m + 1), cs(m), sn(m), y;
+    std::vector<double> H((size_t)(m + 1) * m), h(m + 1), gv(m + 1), cs(m), sn(m), y;
     while (iters < o->max_cycles && o->tol <= resid[iters]) {
       g.precondition(g.r.p);
       const double beta = g.norm(g.r.p);
@@ -220,24 +246,29 @@ extern "C" int mxg_gmres(mxg_h2_t A, mxg_ulv_t M, const mxg_gmres_opts_t* o, con
       int k = 0;
       for (int i = 0; i < m; i++) {
         double* vi = g.V.p + (size_t)i * n;
+        double* h1d = g.hd.p;
+        double* h2d = g.hd.p + (m + 1);
+        double* nrd = g.hd.p + 2 * (m + 1);
         g.matvec(vi, g.w.p, o->inner_precision, Ain);
         g.precondition(g.w.p);
-        g.dots(g.V.p, i + 1, g.w.p, h.data());
-        g.sub_vh(i + 1, h.data());
+        g.dots_dev(g.V.p, i + 1, g.w.p, h1d);
+        g.sub_vh_dev(i + 1, h1d);
         if (!ref) {
-          g.dots(g.V.p, i + 1, g.w.p, h2.data());
-          g.sub_vh(i + 1, h2.data());
-          for (int c = 0; c <= i; c++) h[c] += h2[c];
+          g.dots_dev(g.V.p, i + 1, g.w.p, h2d);
+          g.sub_vh_dev(i + 1, h2d);
         }
-        const double hn = g.norm(g.w.p);
+        g.dots_dev(g.w.p, 1, g.w.p, nrd);
+        hipLaunchKernelGGL(k_scale_rsqrt, dim3(g.grid()), dim3(256), 0, g.s, g.w.p, nrd, n, g.V.p + (size_t)(i + 1) * n);
+        MXG_HIP(hipGetLastError());
+        // one transfer per step: h, h2 and |w|^2
+        MXG_HIP(hipMemcpyAsync(g.hpin, g.hd.p, (2 * (m + 1) + 1) * sizeof(double), hipMemcpyDeviceToHost, g.s));
+        MXG_HIP(hipStreamSynchronize(g.s));
+        for (int c = 0; c <= i; c++) h[c] = g.hpin[c] + (ref ? 0. : g.hpin[(m + 1) + c]);
+        const double hn = std::sqrt(std::max(g.hpin[2 * (m + 1)], 0.));
         for (int c = 0; c <= i; c++) H[c + (size_t)i * (m + 1)] = h[c];
         H[(i + 1) + (size_t)i * (m + 1)] = hn;
         k = i + 1;
         inner_total++;
-        if (hn > 0.) {
-          hipLaunchKernelGGL(k_scale, dim3(g.grid()), dim3(256), 0, g.s, g.w.p, 1. / hn, n, g.V.p + (size_t)(i + 1) * n);
-          MXG_HIP(hipGetLastError());
-        }
         if (!ref) {
           // Givens update of column i and of the residual estimate
           double* col = &H[(size_t)i * (m + 1)];

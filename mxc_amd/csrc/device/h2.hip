@@ -576,11 +576,27 @@ void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y) {
 template void h2_matvec_levels<double>(mxg_h2_s*, const double*, double*);
 template void h2_matvec_levels<float>(mxg_h2_s*, const float*, float*);
 
+void make_shadow32(mxg_h2_s* h);
+
+void h2_apply(mxg_h2_s* h, const double* x, double* y, int precision) {
+  for (long long l = 0; l <= h->L; l++) MXG_REQUIRE(h->lv[l].built, "H2 matrix not fully constructed");
+  MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32, "unknown precision");
+  hipStream_t s = h->ctx->stream;
+  if (precision == MXG_FP64) {
+    h2_matvec_levels<double>(h, x, y);
+  } else {
+    make_shadow32(h);
+    convert(h->x32.p, x, h->n, s);
+    h2_matvec_levels<float>(h, h->x32.p, h->y32.p);
+    convert(y, h->y32.p, h->n, s);
+  }
+}
+
 }  // namespace mxg
 
 using namespace mxg;
 
-namespace {
+namespace mxg {
 
 void make_shadow32(mxg_h2_s* h) {
   if (h->shadow32) return;
@@ -609,7 +625,7 @@ void make_shadow32(mxg_h2_s* h) {
   h->shadow32 = true;
 }
 
-}  // namespace
+}  // namespace mxg
 
 extern "C" {
 
@@ -837,23 +853,9 @@ int mxg_h2_matvec(mxg_h2_t h, const double* d_x, double* d_y, long long nrhs, lo
   return guarded([&] {
     MXG_REQUIRE(h && d_x && d_y, "null argument");
     MXG_REQUIRE(nrhs >= 1 && ld >= h->n, "bad nrhs/ld");
-    for (long long l = 0; l <= h->L; l++) MXG_REQUIRE(h->lv[l].built, "H2 matrix not fully constructed");
-    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32, "unknown precision");
     MXG_HIP(hipSetDevice(h->ctx->device));
-    hipStream_t s = h->ctx->stream;
-    if (precision == MXG_FP32) make_shadow32(h);
-    for (long long c = 0; c < nrhs; c++) {
-      const double* x = d_x + c * ld;
-      double* y = d_y + c * ld;
-      if (precision == MXG_FP64) {
-        h2_matvec_levels<double>(h, x, y);
-      } else {
-        convert(h->x32.p, x, h->n, s);
-        h2_matvec_levels<float>(h, h->x32.p, h->y32.p);
-        convert(y, h->y32.p, h->n, s);
-      }
-    }
-    MXG_HIP(hipStreamSynchronize(s));
+    for (long long c = 0; c < nrhs; c++) h2_apply(h, d_x + c * ld, d_y + c * ld, precision);
+    MXG_HIP(hipStreamSynchronize(h->ctx->stream));
   });
 }
 

@@ -95,6 +95,11 @@ void copy2d_batched(const std::vector<const T*>& src, const std::vector<T*>& dst
                     const std::vector<int>& n, const std::vector<int>& lds, const std::vector<int>& ldd, hipStream_t s);
 template <class T>
 void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y);
+// y = A x on the context stream without a host synchronization (FP64
+// vectors; MXG_FP32 applies the FP32 copy of the operator)
+void h2_apply(mxg_h2_s* h, const double* x, double* y, int precision);
+// x <- M^{-1} x on the context stream without a host synchronization
+void ulv_apply(mxg_ulv_s* f, double* x);
 
 // ---- block-sparse products (bgemv.hip) ----
 // bfloat16 storage: the upper half of the FP32 bit pattern

@@ -451,6 +451,10 @@ struct mxg_ulv_s {
   std::unique_ptr<mxg::UlvBase> impl;
 };
 
+namespace mxg {
+void ulv_apply(mxg_ulv_s* f, double* x) { f->impl->solve(x); }
+}  // namespace mxg
+
 using namespace mxg;
 
 extern "C" {
