@@ -56,6 +56,18 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q))
 
 
+def measured_traffic():
+    """HBM bytes per launch of the FP32 product launch, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE passes (None when absent)."""
+    path = os.path.join(ROOT, "profiles", "pair_gemv_matvec_fp32_traffic.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return float(json.loads(block)["traffic_bytes"])
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def log(msg):
     """Progress on stderr; stdout carries only the JSON line."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -218,7 +230,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
             "algorithmic_bytes_per_launch": bytes_prod_fp32,
-            "traffic": None,
+            "traffic": measured_traffic(),
         },
     }
     if not args.no_cpu_baseline:

@@ -59,12 +59,17 @@ __global__ void __launch_bounds__(256) k_dots(const double* __restrict__ V, int 
   }
 }
 
-__global__ void k_sum_parts(const double* __restrict__ part, int nblk, int ncols, double* __restrict__ out) {
-  const int c = threadIdx.x;
+// out[c] = sum_b part[b * ncols + c]: one wave per column, lanes over the
+// blocks in a fixed stride, then a fixed-shape wave reduction
+__global__ void __launch_bounds__(256) k_sum_parts(const double* __restrict__ part, int nblk, int ncols,
+                                                   double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c >= ncols) return;
   double s = 0.;
-  for (int b = 0; b < nblk; b++) s += part[(long long)b * ncols + c];
-  out[c] = s;
+  for (int b = lane; b < nblk; b += 64) s += part[(long long)b * ncols + c];
+  s = wave_sum(s);
+  if (lane == 0) out[c] = s;
 }
 
 // y = x / sqrt(nrm2[0]) (zero when nrm2 is not positive)
@@ -139,7 +144,7 @@ struct Gmres {
   void dots_dev(const double* Vp, int ncols, const double* wp, double* out) {
     hipLaunchKernelGGL(k_dots, dim3(nblk), dim3(256), 0, s, Vp, ncols, n, n, wp, part.p);
     MXG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64 * ((ncols + 63) / 64)), 0, s, part.p, nblk, ncols, out);
+    hipLaunchKernelGGL(k_sum_parts, dim3((ncols + 3) / 4), dim3(256), 0, s, part.p, nblk, ncols, out);
     MXG_HIP(hipGetLastError());
   }
 
