@@ -77,6 +77,10 @@ int mxg_gen_block(mxg_ctx_t ctx, mxg_kernel_t k, long long m, long long n, const
 /* Dense FP64 check y = A x, matrix free (main.cpp:83-84,114 with kernel.cpp:25-35). */
 int mxg_dense_matvec(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n, const double* d_x,
                      double* d_y);
+/* Y = K X for nrhs columns (leading dimensions ldx / ldy): every kernel entry
+ * is evaluated once per group of up to 8 columns. */
+int mxg_dense_matvec_batch(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n, const double* d_X,
+                           long long ldx, double* d_Y, long long ldy, long long nrhs);
 
 /* ---- H^2 matrix: replaces createSpMatrixDesc/createDeviceCsr/createMatrixDesc and
  *      the numeric work of H2Matrix::construct (h2matrix.cpp:72-249). ---- */
@@ -154,6 +158,16 @@ typedef struct {
  * iters_out: cycles run; inner_out: total inner iterations. */
 int mxg_gmres(mxg_h2_t A, mxg_ulv_t M, const mxg_gmres_opts_t* opts, const double* d_b, double* d_x,
               double* resid, long long* iters_out, long long* inner_out);
+/* Batch of nrhs (1..64) independent right-hand sides solved in lockstep: each
+ * has its own Krylov basis and stopping test, the operator and
+ * preconditioner applications of a step serve all of them (the H^2 and ULV
+ * data are streamed once per step for up to 8 of them). B and X are n x nrhs
+ * with leading dimensions ldb / ldx. resid: nrhs rows of max_cycles + 1;
+ * iters_out / inner_out: nrhs entries. No counterpart in the reference, whose
+ * solveGMRES takes one right-hand side. */
+int mxg_gmres_batch(mxg_h2_t A, mxg_ulv_t M, const mxg_gmres_opts_t* opts, long long nrhs, const double* d_B,
+                    long long ldb, double* d_X, long long ldx, double* resid, long long* iters_out,
+                    long long* inner_out);
 
 /* ---- kernel timing (HIP events on the context stream) ----
  * While enabled, every launch of the instrumented kernel classes is bracketed

@@ -85,6 +85,11 @@ int mxc_solver_precondition(mxc_solver_t s, double* d_x);
  * (mxc_solver_device) used for the inner iterations. */
 int mxc_solver_gmres(mxc_solver_t A, mxc_solver_t M, const mxg_gmres_opts_t* o, const double* d_b, double* d_x,
                      double* resid, long long* iters, long long* inner);
+/* nrhs right-hand sides in lockstep (mxg_gmres_batch): B, X n x nrhs with
+ * leading dimensions ldb / ldx; resid nrhs x (max_cycles + 1); iters, inner
+ * nrhs entries */
+int mxc_solver_gmres_batch(mxc_solver_t A, mxc_solver_t M, const mxg_gmres_opts_t* o, long long nrhs, const double* d_B,
+                           long long ldb, double* d_X, long long ldx, double* resid, long long* iters, long long* inner);
 /* dense FP64 check y = K x over the problem's bodies */
 int mxc_dense_matvec(mxc_problem_t p, mxg_kernel_t k, const double* d_x, double* d_y);
 

@@ -87,6 +87,7 @@ def gpu() -> C.CDLL:
             "mxg_gemm": [P, c_int, c_int, c_int, c_ll, c_ll, c_ll, c_d, P, c_ll, P, c_ll, c_d, P, c_ll],
             "mxg_gen_block": [P, MxgKernel, c_ll, c_ll, P, P, P, c_ll],
             "mxg_dense_matvec": [P, MxgKernel, P, c_ll, P, P],
+            "mxg_dense_matvec_batch": [P, MxgKernel, P, c_ll, P, c_ll, P, c_ll, c_ll],
             "mxg_h2_matvec": [P, P, P, c_ll, c_ll, c_int],
             "mxg_h2_download": [P, c_ll, c_int, P, c_ll],
             "mxg_ulv_solve": [P, P, c_ll, c_ll],
@@ -132,6 +133,7 @@ def h2() -> C.CDLL:
             "mxc_solver_factorize": [P, c_int, P],
             "mxc_solver_precondition": [P, P],
             "mxc_solver_gmres": [P, P, C.POINTER(MxgGmresOpts), P, P, P, C.POINTER(c_ll), C.POINTER(c_ll)],
+            "mxc_solver_gmres_batch": [P, P, C.POINTER(MxgGmresOpts), c_ll, P, c_ll, P, c_ll, P, P, P],
             "mxc_dense_matvec": [P, MxgKernel, P, P],
         }
         for name, args in sigs.items():

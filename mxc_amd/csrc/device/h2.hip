@@ -502,7 +502,9 @@ struct Ops<float> {
 };
 
 // Device jobs of one precision: the upward pass per level, and one list with
-// the far terms of every level and the near terms of the leaf level.
+// the far terms of every level and the near terms of the leaf level. The
+// vectors of a level hold MAX_RHS right-hand sides, lenZ (or n, or the term
+// count) apart.
 template <class T>
 void ensure_matvec_jobs(mxg_h2_s* h) {
   constexpr int pi = sizeof(T) == 8 ? 0 : 1;
@@ -516,79 +518,111 @@ void ensure_matvec_jobs(mxg_h2_s* h) {
   }
   h->lv[L].tbaseN = tlen;
   tlen += h->lv[L].lenTN;
+  h->tlen = std::max<long long>(tlen, 1);
   DevBuf<T>& tb = Ops<T>::t(h);
-  tb.alloc((size_t)std::max<long long>(tlen, 1));
-  if constexpr (pi == 0) h->x64.reserve((size_t)h->n);
+  tb.alloc((size_t)h->tlen * MAX_RHS);
+  if constexpr (pi == 0) h->x64.reserve((size_t)h->n * MAX_RHS);
   std::vector<PairJob> prod;
   for (long long l = 0; l <= L; l++) {
     Level& Lv = h->lv[l];
     std::vector<PairJob> up;
     const T* xin = (l == L) ? Ops<T>::x(h) : Ops<T>::Z(h, l + 1);
-    pair_jobs_append<T, T>(up, Lv.up_t, Ops<T>::QT(Lv), (long long)Lv.offQT[Lv.nb], xin, Lv.lenX, Ops<T>::Z(h, l),
-                        std::max<long long>(Lv.lenZ, 1));
+    const long long lzin = (l == L) ? h->n : h->lv[l + 1].lenZ;
+    if (Lv.lenZ > 0) {
+      pair_jobs_append<T, T>(up, Lv.up_t, Ops<T>::QT(Lv), (long long)Lv.offQT[Lv.nb], xin, Lv.lenX, lzin, Ops<T>::Z(h, l),
+                             Lv.lenZ, Lv.lenZ);
+      pair_jobs_append<T, T>(prod, Lv.far_t, Ops<T>::C(Lv), (long long)Lv.C.n, Ops<T>::Z(h, l), Lv.lenZ, Lv.lenZ,
+                             tb.p + Lv.tbaseF, Lv.lenTF, h->tlen);
+    }
     pair_list_upload(Lv.up_pl[pi], up, s);
-    pair_jobs_append<T, T>(prod, Lv.far_t, Ops<T>::C(Lv), (long long)Lv.C.n, Ops<T>::Z(h, l), std::max<long long>(Lv.lenZ, 1),
-                        tb.p + Lv.tbaseF, Lv.lenTF);
   }
   Level& Lf = h->lv[L];
-  pair_jobs_append<T, T>(prod, Lf.near_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, tb.p + Lf.tbaseN, Lf.lenTN);
+  pair_jobs_append<T, T>(prod, Lf.near_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, h->n, tb.p + Lf.tbaseN,
+                         Lf.lenTN, h->tlen);
   pair_list_upload(h->prod_pl[pi], prod, s);
   MXG_HIP(hipStreamSynchronize(s));
   h->jobs_ready[pi] = true;
 }
 
+template <class Td, class Ts>
+__global__ void k_convert2d(Td* __restrict__ dst, long long ldd, const Ts* __restrict__ src, long long lds, long long n,
+                            int ncols) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int c = 0; c < ncols; c++) dst[r + c * ldd] = (Td)src[r + c * lds];
+}
+
 }  // namespace
 
-// y = A x on the device. x and y are FP64 (T = double) or the FP32 copies
-// h->x32 / h->y32 (T = float); x is staged into the internal input buffer.
+template <class Td, class Ts>
+void convert2d(Td* dst, long long ldd, const Ts* src, long long lds, long long n, int ncols, hipStream_t s) {
+  if (n == 0 || ncols == 0) return;
+  hipLaunchKernelGGL((k_convert2d<Td, Ts>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, dst, ldd, src, lds, n, ncols);
+  MXG_HIP(hipGetLastError());
+}
+template void convert2d<double, double>(double*, long long, const double*, long long, long long, int, hipStream_t);
+template void convert2d<float, double>(float*, long long, const double*, long long, long long, int, hipStream_t);
+template void convert2d<double, float>(double*, long long, const float*, long long, long long, int, hipStream_t);
+
+// Y = A X for nrhs <= MAX_RHS right-hand sides staged in the precision's
+// input buffer (n apart); Y has leading dimension ldy.
 template <class T>
-void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y) {
+void h2_matvec_levels(mxg_h2_s* h, T* y, long long ldy, int nrhs) {
   hipStream_t s = h->ctx->stream;
   const long long L = h->L;
   constexpr bool f64 = sizeof(T) == 8;
   constexpr int pi = f64 ? 0 : 1;
   ensure_matvec_jobs<T>(h);
-  T* xb = Ops<T>::x(h);
-  if (x != xb) MXG_HIP(hipMemcpyAsync(xb, x, h->n * sizeof(T), hipMemcpyDeviceToDevice, s));
   T* tb = Ops<T>::t(h).p;
+  const long long lt = h->tlen;
   // upward pass: z_i = Q_i[:, :k]^T x_i
   for (long long l = L; l >= 0; l--) {
     if (h->lv[l].lenZ == 0) continue;
     ProfScope prof(h->ctx, MXG_PROF_MATVEC_UPPER);
-    pair_gemv<T, T>(h->lv[l].up_pl[pi], s);
+    pair_gemv<T, T>(h->lv[l].up_pl[pi], nrhs, s);
   }
   // every far term of every level and the near terms of the leaf, one launch
   {
     ProfScope prof(h->ctx, f64 ? MXG_PROF_MATVEC_LEAF_FP64 : MXG_PROF_MATVEC_LEAF_FP32);
-    pair_gemv<T, T>(h->prod_pl[pi], s);
+    pair_gemv<T, T>(h->prod_pl[pi], nrhs, s);
   }
   // downward pass: w = w_in + far terms, y = Q w (+ near terms at the leaf)
   for (long long l = 0; l <= L; l++) {
     Level& Lv = h->lv[l];
     const bool leaf = (l == L);
     T* out = leaf ? y : Ops<T>::W(h, l + 1);
+    const long long ldo = leaf ? ldy : h->lv[l + 1].lenZ;
+    if (!leaf && ldo == 0) continue;
     ProfScope prof(h->ctx, MXG_PROF_MATVEC_UPPER);
-    comb_down<T>(Lv, Lv.vecQ[pi], Lv.d_toffF.p, Lv.d_toffN.p, Ops<T>::Q(Lv), Ops<T>::W(h, l), tb + Lv.tbaseF,
-                 leaf ? tb + Lv.tbaseN : nullptr, out, s);
+    comb_down<T>(Lv, Lv.vecQ[pi], Lv.d_toffF.p, Lv.d_toffN.p, Ops<T>::Q(Lv), Ops<T>::W(h, l), std::max<long long>(Lv.lenZ, 1),
+                 tb + Lv.tbaseF, leaf ? tb + Lv.tbaseN : nullptr, lt, out, ldo, nrhs, s);
   }
 }
 
-template void h2_matvec_levels<double>(mxg_h2_s*, const double*, double*);
-template void h2_matvec_levels<float>(mxg_h2_s*, const float*, float*);
+template void h2_matvec_levels<double>(mxg_h2_s*, double*, long long, int);
+template void h2_matvec_levels<float>(mxg_h2_s*, float*, long long, int);
 
 void make_shadow32(mxg_h2_s* h);
 
-void h2_apply(mxg_h2_s* h, const double* x, double* y, int precision) {
+void h2_apply(mxg_h2_s* h, const double* x, long long ldx, double* y, long long ldy, long long nrhs, int precision) {
   for (long long l = 0; l <= h->L; l++) MXG_REQUIRE(h->lv[l].built, "H2 matrix not fully constructed");
   MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32, "unknown precision");
+  MXG_REQUIRE(nrhs >= 1 && ldx >= h->n && ldy >= h->n, "bad right-hand sides");
   hipStream_t s = h->ctx->stream;
-  if (precision == MXG_FP64) {
-    h2_matvec_levels<double>(h, x, y);
-  } else {
-    make_shadow32(h);
-    convert(h->x32.p, x, h->n, s);
-    h2_matvec_levels<float>(h, h->x32.p, h->y32.p);
-    convert(y, h->y32.p, h->n, s);
+  if (precision == MXG_FP32) make_shadow32(h);
+  const long long n = h->n;
+  for (long long c0 = 0; c0 < nrhs; c0 += MAX_RHS) {
+    const int nr = (int)std::min<long long>(MAX_RHS, nrhs - c0);
+    if (precision == MXG_FP64) {
+      ensure_matvec_jobs<double>(h);
+      convert2d(h->x64.p, n, x + c0 * ldx, ldx, n, nr, s);
+      h2_matvec_levels<double>(h, y + c0 * ldy, ldy, nr);
+    } else {
+      ensure_matvec_jobs<float>(h);
+      convert2d(h->x32.p, n, x + c0 * ldx, ldx, n, nr, s);
+      h2_matvec_levels<float>(h, h->y32.p, n, nr);
+      convert2d(y + c0 * ldy, ldy, h->y32.p, n, n, nr, s);
+    }
   }
 }
 
@@ -615,11 +649,11 @@ void make_shadow32(mxg_h2_s* h) {
       Lv.A32.alloc(Lv.A.n);
       convert(Lv.A32.p, Lv.A.p, Lv.A.n, s);
     }
-    h->Z32[l].alloc(std::max<long long>(Lv.lenZ, 1));
-    h->W32[l].alloc_zero(std::max<long long>(Lv.lenZ, 1), s);
+    h->Z32[l].alloc(std::max<long long>(Lv.lenZ, 1) * MAX_RHS);
+    h->W32[l].alloc_zero(std::max<long long>(Lv.lenZ, 1) * MAX_RHS, s);
   }
-  h->x32.alloc(h->n);
-  h->y32.alloc(h->n);
+  h->x32.alloc(h->n * MAX_RHS);
+  h->y32.alloc(h->n * MAX_RHS);
   h->jobs_ready[1] = false;
   MXG_HIP(hipStreamSynchronize(s));
   h->shadow32 = true;
@@ -806,8 +840,8 @@ int mxg_h2_construct_level(mxg_h2_t h, long long l, long long nb, const long lon
     upload_indices(h, Lv);
     build_matvec_lists(h, Lv, !low);
     h->jobs_ready[0] = h->jobs_ready[1] = false;
-    h->Z[l].alloc_zero(std::max<long long>(Lv.lenZ, 1), s);
-    h->W[l].alloc_zero(std::max<long long>(Lv.lenZ, 1), s);
+    h->Z[l].alloc_zero(std::max<long long>(Lv.lenZ, 1) * MAX_RHS, s);
+    h->W[l].alloc_zero(std::max<long long>(Lv.lenZ, 1) * MAX_RHS, s);
     MXG_HIP(hipStreamSynchronize(s));
     Lv.built = true;
     h->shadow32 = false;
@@ -854,7 +888,7 @@ int mxg_h2_matvec(mxg_h2_t h, const double* d_x, double* d_y, long long nrhs, lo
     MXG_REQUIRE(h && d_x && d_y, "null argument");
     MXG_REQUIRE(nrhs >= 1 && ld >= h->n, "bad nrhs/ld");
     MXG_HIP(hipSetDevice(h->ctx->device));
-    for (long long c = 0; c < nrhs; c++) h2_apply(h, d_x + c * ld, d_y + c * ld, precision);
+    h2_apply(h, d_x, ld, d_y, ld, nrhs, precision);
     MXG_HIP(hipStreamSynchronize(h->ctx->stream));
   });
 }

@@ -31,8 +31,11 @@ struct PairJob {
   const void* B;
   const void* v;
   void* out;
+  long long ldv, ldo;  // strides (elements) between right-hand sides of v and out
   int n, m, ld, vec;
 };
+// right-hand sides one launch of the solve-phase kernels carries at most
+constexpr int MAX_RHS = 8;
 struct PairList {
   DevBuf<PairJob> jobs;
   long long count = 0;
@@ -86,6 +89,7 @@ struct mxg_h2_s {
   mxg::DevBuf<float> t32;
   mxg::PairList prod_pl[2];
   bool jobs_ready[2] = {false, false};
+  long long tlen = 1;  // term count of the matvec (the stride between right-hand sides)
 };
 
 namespace mxg {
@@ -93,38 +97,47 @@ namespace mxg {
 template <class T>
 void copy2d_batched(const std::vector<const T*>& src, const std::vector<T*>& dst, const std::vector<int>& m,
                     const std::vector<int>& n, const std::vector<int>& lds, const std::vector<int>& ldd, hipStream_t s);
+// y = A x for nrhs <= MAX_RHS right-hand sides: x is the internal staging
+// buffer of the precision (n x MAX_RHS), y has leading dimension ldy
 template <class T>
-void h2_matvec_levels(mxg_h2_s* h, const T* x, T* y);
-// y = A x on the context stream without a host synchronization (FP64
-// vectors; MXG_FP32 applies the FP32 copy of the operator)
-void h2_apply(mxg_h2_s* h, const double* x, double* y, int precision);
-// x <- M^{-1} x on the context stream without a host synchronization
-void ulv_apply(mxg_ulv_s* f, double* x);
+void h2_matvec_levels(mxg_h2_s* h, T* y, long long ldy, int nrhs);
+// Y = A X (n x nrhs, leading dimensions ldx / ldy, FP64) on the context stream
+// without a host synchronization; MXG_FP32 applies the FP32 copy of the operator
+void h2_apply(mxg_h2_s* h, const double* x, long long ldx, double* y, long long ldy, long long nrhs, int precision);
+// dst[r + c ldd] = src[r + c lds], r < n, c < ncols (with conversion)
+template <class Td, class Ts>
+void convert2d(Td* dst, long long ldd, const Ts* src, long long lds, long long n, int ncols, hipStream_t s);
+// X <- M^{-1} X (n x nrhs, leading dimension ldx) without a host synchronization
+void ulv_apply(mxg_ulv_s* f, double* x, long long ldx, long long nrhs);
 
 // ---- block-sparse products (bgemv.hip) ----
 // bfloat16 storage: the upper half of the FP32 bit pattern
 typedef unsigned short bf16_t;
 // Appends the jobs of tasks over the arrays B (nB elements, stored as S), v
-// (nv) and out (nout), after checking every task against those sizes.
+// (nv per right-hand side, ldv apart) and out (nout per right-hand side, ldo
+// apart), after checking every task against those sizes.
 template <class T, class S>
 void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& tasks, const S* B, long long nB, const T* v,
-                      long long nv, T* out, long long nout);
+                      long long nv, long long ldv, T* out, long long nout, long long ldo);
 void pair_list_upload(PairList& pl, const std::vector<PairJob>& jobs, hipStream_t s);
-// out[r] = sum_c B[r + c*ld] v[c] for every job, one workgroup per job,
-// blocks stored as S, arithmetic in T
+// out[r + k ldo] = sum_c B[r + c*ld] v[c + k ldv] for every job and the
+// right-hand sides k < nrhs (<= MAX_RHS), one workgroup per job, blocks
+// stored as S, arithmetic in T
 template <class T, class S>
-void pair_gemv(const PairList& pl, hipStream_t s);
+void pair_gemv(const PairList& pl, int nrhs, hipStream_t s);
 // FP32 -> bfloat16, round to nearest even
 void to_bf16(bf16_t* dst, const float* src, size_t n, hipStream_t s);
-// w_i = w_in_i + sum_far tF, y_i = Q_i[:, :k] w_i (+ sum_near tN when tN != nullptr)
+// w_i = w_in_i + sum_far tF, y_i = Q_i[:, :k] w_i (+ sum_near tN when tN !=
+// nullptr); ld*: strides between right-hand sides (tF and tN share ldt)
 template <class T>
 void comb_down(const Level& Lv, bool vec, const long long* d_toffF, const long long* d_toffN, const T* Q, const T* w_in,
-               const T* tF, const T* tN, T* y, hipStream_t s);
+               long long ldw, const T* tF, const T* tN, long long ldt, T* y, long long ldy, int nrhs, hipStream_t s);
 // z_i = x_i[:s] - sum_near t
 template <class T>
-void comb_fwd(const Level& Lv, const long long* d_toff, const T* x, const T* t, T* z, hipStream_t s);
+void comb_fwd(const Level& Lv, const long long* d_toff, const T* x, long long ldx, const T* t, long long ldt, T* z,
+              long long ldz, int nrhs, hipStream_t s);
 // y_i = U_i [w_i ; x_i[s:] - sum_near t], U stored as S
 template <class T, class S>
-void comb_bwd(const Level& Lv, bool vec, const long long* d_toff, const S* U, const T* x, const T* w, const T* t, T* y,
-              hipStream_t s);
+void comb_bwd(const Level& Lv, bool vec, const long long* d_toff, const S* U, const T* x, long long ldx, const T* w,
+              long long ldw, const T* t, long long ldt, T* y, long long ldy, int nrhs, hipStream_t s);
 }  // namespace mxg

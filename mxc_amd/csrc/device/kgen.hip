@@ -34,21 +34,25 @@ __global__ void __launch_bounds__(256) k_gen(const GenTask* __restrict__ tasks, 
   }
 }
 
-// Dense check: partial sums of y over column chunks, then a fixed-order
-// reduction (deterministic; no atomics).
+// Dense check: partial sums of Y = K X over column chunks, then a
+// fixed-order reduction (deterministic; no atomics). Each kernel entry is
+// evaluated once and applied to the NR right-hand sides of the launch.
 constexpr int DM_ROWS = 256, DM_CHUNK = 8192;
 
+template <int NR>
 __global__ void __launch_bounds__(256) k_dense_partial(const double* __restrict__ bodies, const double* __restrict__ x,
-                                                       long long n, int kid, double diag, double alpha,
+                                                       long long ldx, long long n, int kid, double diag, double alpha,
                                                        double* __restrict__ part) {
   __shared__ double sp[256 * 3];
-  __shared__ double sx[256];
+  __shared__ double sx[NR][256];
   long long i = (long long)blockIdx.x * DM_ROWS + threadIdx.x;
   long long j0 = (long long)blockIdx.y * DM_CHUNK;
   long long j1 = min(j0 + DM_CHUNK, n);
   double p[3] = {0., 0., 0.};
   if (i < n) { p[0] = bodies[3 * i]; p[1] = bodies[3 * i + 1]; p[2] = bodies[3 * i + 2]; }
-  double acc = 0.;
+  double acc[NR];
+#pragma unroll
+  for (int q = 0; q < NR; q++) acc[q] = 0.;
   for (long long jt = j0; jt < j1; jt += 256) {
     int cnt = (int)min((long long)256, j1 - jt);
     __syncthreads();
@@ -57,20 +61,30 @@ __global__ void __launch_bounds__(256) k_dense_partial(const double* __restrict_
       sp[3 * threadIdx.x] = bodies[3 * j];
       sp[3 * threadIdx.x + 1] = bodies[3 * j + 1];
       sp[3 * threadIdx.x + 2] = bodies[3 * j + 2];
-      sx[threadIdx.x] = x[j];
+#pragma unroll
+      for (int q = 0; q < NR; q++) sx[q][threadIdx.x] = x[j + q * ldx];
     }
     __syncthreads();
-    for (int c = 0; c < cnt; c++) acc += kernel_eval(kid, diag, alpha, dist2(p, &sp[3 * c])) * sx[c];
+    for (int c = 0; c < cnt; c++) {
+      const double kv = kernel_eval(kid, diag, alpha, dist2(p, &sp[3 * c]));
+#pragma unroll
+      for (int q = 0; q < NR; q++) acc[q] += kv * sx[q][c];
+    }
   }
-  if (i < n) part[(long long)blockIdx.y * n + i] = acc;
+  if (i < n)
+#pragma unroll
+    for (int q = 0; q < NR; q++) part[((long long)blockIdx.y * NR + q) * n + i] = acc[q];
 }
 
-__global__ void k_dense_reduce(const double* __restrict__ part, long long n, int nchunks, double* __restrict__ y) {
+__global__ void k_dense_reduce(const double* __restrict__ part, long long n, int nchunks, int nr, double* __restrict__ y,
+                               long long ldy) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double s = 0.;
-  for (int c = 0; c < nchunks; c++) s += part[(long long)c * n + i];
-  y[i] = s;
+  for (int q = 0; q < nr; q++) {
+    double s = 0.;
+    for (int c = 0; c < nchunks; c++) s += part[((long long)c * nr + q) * n + i];
+    y[i + q * ldy] = s;
+  }
 }
 
 }  // namespace
@@ -123,23 +137,40 @@ int mxg_gen_block(mxg_ctx_t ctx, mxg_kernel_t k, long long m, long long n, const
   });
 }
 
-int mxg_dense_matvec(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n, const double* d_x, double* d_y) {
+int mxg_dense_matvec_batch(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n, const double* d_X,
+                           long long ldx, double* d_Y, long long ldy, long long nrhs) {
   return guarded([&] {
     MXG_REQUIRE(ctx, "ctx is null");
-    MXG_REQUIRE(n >= 0, "negative size");
-    if (n == 0) return;
-    MXG_REQUIRE(d_bodies && d_x && d_y, "null pointer");
+    MXG_REQUIRE(n >= 0 && nrhs >= 0, "negative size");
+    if (n == 0 || nrhs == 0) return;
+    MXG_REQUIRE(d_bodies && d_X && d_Y, "null pointer");
+    MXG_REQUIRE(ldx >= n && ldy >= n, "bad leading dimensions");
     int nchunks = (int)ceil_div(n, DM_CHUNK);
     MXG_REQUIRE(ceil_div(n, DM_ROWS) < (1u << 31) && nchunks < 65536, "dense check size too large");
     DevBuf<double> part;
-    part.alloc((size_t)nchunks * n);
-    hipLaunchKernelGGL(k_dense_partial, dim3(ceil_div(n, DM_ROWS), nchunks), dim3(256), 0, ctx->stream, d_bodies, d_x, n, k.id,
-                       k.diag, k.alpha, part.p);
-    MXG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_dense_reduce, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, part.p, n, nchunks, d_y);
-    MXG_HIP(hipGetLastError());
+    part.alloc((size_t)nchunks * n * std::min<long long>(nrhs, 8));
+    for (long long c0 = 0; c0 < nrhs;) {
+      const int nr = nrhs - c0 >= 8 ? 8 : nrhs - c0 >= 4 ? 4 : nrhs - c0 >= 2 ? 2 : 1;
+      const dim3 g(ceil_div(n, DM_ROWS), nchunks), b(256);
+      const double* x = d_X + c0 * ldx;
+      switch (nr) {
+        case 8: hipLaunchKernelGGL(k_dense_partial<8>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, part.p); break;
+        case 4: hipLaunchKernelGGL(k_dense_partial<4>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, part.p); break;
+        case 2: hipLaunchKernelGGL(k_dense_partial<2>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, part.p); break;
+        default: hipLaunchKernelGGL(k_dense_partial<1>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, part.p); break;
+      }
+      MXG_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_dense_reduce, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, part.p, n, nchunks, nr,
+                         d_Y + c0 * ldy, ldy);
+      MXG_HIP(hipGetLastError());
+      c0 += nr;
+    }
     MXG_HIP(hipStreamSynchronize(ctx->stream));
   });
+}
+
+int mxg_dense_matvec(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n, const double* d_x, double* d_y) {
+  return mxg_dense_matvec_batch(ctx, k, d_bodies, n, d_x, n, d_y, n, 1);
 }
 
 }  // extern "C"

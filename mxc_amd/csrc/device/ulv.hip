@@ -52,7 +52,7 @@ constexpr size_t PAIR_BUDGET = size_t(1) << 28;
 struct UlvBase {
   virtual ~UlvBase() = default;
   virtual void factorize() = 0;
-  virtual void solve(double* x) = 0;
+  virtual void solve(double* x, long long ldx, long long nrhs) = 0;
   virtual void info(int* out) = 0;
 };
 
@@ -75,7 +75,8 @@ struct UlvImpl : UlvBase {
     bool vecM = false;
   };
   std::vector<Lv> lv;
-  DevBuf<T> vin, tbuf;
+  DevBuf<T> vin, tbuf;  // n and tlen entries per right-hand side
+  long long tlen = 1;
 
   explicit UlvImpl(mxg_h2_s* hh) : h(hh) {
     hipStream_t s = h->ctx->stream;
@@ -95,8 +96,8 @@ struct UlvImpl : UlvBase {
         F.offRR[i + 1] = F.offRR[i] + r * r;
       }
       F.RRinv.alloc(std::max<long long>(F.offRR.back(), 1));
-      F.X.alloc(std::max<long long>(H.lenX, 1));
-      F.Z.alloc(std::max<long long>(H.lenZ, 1));
+      F.X.alloc(std::max<long long>(H.lenX, 1) * MAX_RHS);
+      F.Z.alloc(std::max<long long>(H.lenZ, 1) * MAX_RHS);
       F.info.alloc_zero(std::max<long long>(H.nb, 1), s);
       if constexpr (kCopy) {
         F.As.alloc(std::max<size_t>(F.A.n, 1));
@@ -104,7 +105,7 @@ struct UlvImpl : UlvBase {
         F.Vs.alloc(std::max<size_t>(F.V.n, 1));
       }
     }
-    vin.alloc(h->n);
+    vin.alloc(h->n * MAX_RHS);
     long long tmax = 1;
     for (long long l = 0; l <= h->L; l++) {
       const Level& H = h->lv[l];
@@ -116,7 +117,8 @@ struct UlvImpl : UlvBase {
       }
       tmax = std::max(tmax, std::max(u, b));
     }
-    tbuf.alloc((size_t)tmax);
+    tlen = tmax;
+    tbuf.alloc((size_t)tmax * MAX_RHS);
     for (long long l = 0; l <= h->L; l++) build_solve_lists(l, tmax);
     MXG_HIP(hipStreamSynchronize(s));
   }
@@ -156,10 +158,11 @@ struct UlvImpl : UlvBase {
     }
     const T* y = (l == h->L) ? vin.p : lv[l + 1].Z.p;
     const long long ylen = (l == h->L) ? h->n : std::max<long long>(h->lv[l + 1].lenZ, 1);
+    const long long lx = std::max<long long>(H.lenX, 1), lz = std::max<long long>(H.lenZ, 1);
     std::vector<PairJob> jv, jf, jb;
-    pair_jobs_append<T, S>(jv, vt, Vstore(F), H.offQ[nb], y, ylen, F.X.p, std::max<long long>(H.lenX, 1));
-    pair_jobs_append<T, S>(jf, ft, Astore(F), (long long)F.A.n, F.X.p, std::max<long long>(H.lenX, 1), tbuf.p, tlen);
-    pair_jobs_append<T, S>(jb, bt, Astore(F), (long long)F.A.n, F.Z.p, std::max<long long>(H.lenZ, 1), tbuf.p, tlen);
+    pair_jobs_append<T, S>(jv, vt, Vstore(F), H.offQ[nb], y, ylen, ylen, F.X.p, lx, lx);
+    pair_jobs_append<T, S>(jf, ft, Astore(F), (long long)F.A.n, F.X.p, lx, lx, tbuf.p, tlen, tlen);
+    pair_jobs_append<T, S>(jb, bt, Astore(F), (long long)F.A.n, F.Z.p, lz, lz, tbuf.p, tlen, tlen);
     pair_list_upload(F.v_pl, jv, s);
     pair_list_upload(F.fwd_pl, jf, s);
     pair_list_upload(F.bwd_pl, jb, s);
@@ -404,37 +407,37 @@ struct UlvImpl : UlvBase {
     copy2d_batched<T>(cs, cd, cm, cn, cls, cld, h->ctx->stream);
   }
 
-  void solve(double* x) override {
+  void solve(double* x, long long ldx, long long nrhs) override {
     hipStream_t s = h->ctx->stream;
-    const long long L = h->L;
+    const long long L = h->L, n = h->n;
     T* v = vin.p;
-    if constexpr (sizeof(T) == 8)
-      MXG_HIP(hipMemcpyAsync(v, x, h->n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    else
-      convert(v, x, h->n, s);
-    for (long long l = L; l >= 0; l--) {
-      Level& H = h->lv[l];
-      Lv& F = lv[l];
-      ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_FWD_LEAF : -1);
-      pair_gemv<T, S>(F.v_pl, s);
-      if (H.lenZ > 0) {
-        pair_gemv<T, S>(F.fwd_pl, s);
-        comb_fwd<T>(H, F.d_toffU.p, F.X.p, tbuf.p, F.Z.p, s);
+    for (long long c0 = 0; c0 < nrhs; c0 += MAX_RHS) {
+      const int nr = (int)std::min<long long>(MAX_RHS, nrhs - c0);
+      convert2d(v, n, x + c0 * ldx, ldx, n, nr, s);
+      for (long long l = L; l >= 0; l--) {
+        Level& H = h->lv[l];
+        Lv& F = lv[l];
+        const long long lx = std::max<long long>(H.lenX, 1), lz = std::max<long long>(H.lenZ, 1);
+        ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_FWD_LEAF : -1);
+        pair_gemv<T, S>(F.v_pl, nr, s);
+        if (H.lenZ > 0) {
+          pair_gemv<T, S>(F.fwd_pl, nr, s);
+          comb_fwd<T>(H, F.d_toffU.p, F.X.p, lx, tbuf.p, tlen, F.Z.p, lz, nr, s);
+        }
       }
+      for (long long l = 0; l <= L; l++) {
+        Level& H = h->lv[l];
+        Lv& F = lv[l];
+        const long long lx = std::max<long long>(H.lenX, 1), lz = std::max<long long>(H.lenZ, 1);
+        // w of level l is the output of level l - 1 (stored in this level's Z buffer)
+        T* out = (l == L) ? v : lv[l + 1].Z.p;
+        const long long lo = (l == L) ? n : std::max<long long>(h->lv[l + 1].lenZ, 1);
+        ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_BWD_LEAF : -1);
+        pair_gemv<T, S>(F.bwd_pl, nr, s);
+        comb_bwd<T, S>(H, F.vecM, F.d_toffB.p, Ustore(F), F.X.p, lx, F.Z.p, lz, tbuf.p, tlen, out, lo, nr, s);
+      }
+      convert2d(x + c0 * ldx, ldx, v, n, n, nr, s);
     }
-    for (long long l = 0; l <= L; l++) {
-      Level& H = h->lv[l];
-      Lv& F = lv[l];
-      // w of level l is the output of level l - 1 (stored in this level's Z buffer)
-      T* out = (l == L) ? v : lv[l + 1].Z.p;
-      ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_BWD_LEAF : -1);
-      pair_gemv<T, S>(F.bwd_pl, s);
-      comb_bwd<T, S>(H, F.vecM, F.d_toffB.p, Ustore(F), F.X.p, F.Z.p, tbuf.p, out, s);
-    }
-    if constexpr (sizeof(T) == 8)
-      MXG_HIP(hipMemcpyAsync(x, v, h->n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    else
-      convert(x, v, h->n, s);
   }
 
   void info(int* out) override {
@@ -452,7 +455,7 @@ struct mxg_ulv_s {
 };
 
 namespace mxg {
-void ulv_apply(mxg_ulv_s* f, double* x) { f->impl->solve(x); }
+void ulv_apply(mxg_ulv_s* f, double* x, long long ldx, long long nrhs) { f->impl->solve(x, ldx, nrhs); }
 }  // namespace mxg
 
 using namespace mxg;
@@ -503,7 +506,7 @@ int mxg_ulv_solve(mxg_ulv_t f, double* d_x, long long nrhs, long long ld) {
     MXG_REQUIRE(f && d_x, "null argument");
     MXG_REQUIRE(nrhs >= 1 && ld >= f->h->n, "bad nrhs/ld");
     MXG_HIP(hipSetDevice(f->h->ctx->device));
-    for (long long c = 0; c < nrhs; c++) f->impl->solve(d_x + c * ld);
+    f->impl->solve(d_x, ld, nrhs);
     MXG_HIP(hipStreamSynchronize(f->h->ctx->stream));
   });
 }

@@ -267,6 +267,16 @@ int mxc_solver_gmres(mxc_solver_t A, mxc_solver_t M, const mxg_gmres_opts_t* o, 
   });
 }
 
+int mxc_solver_gmres_batch(mxc_solver_t A, mxc_solver_t M, const mxg_gmres_opts_t* o, long long nrhs, const double* d_B,
+                           long long ldb, double* d_X, long long ldx, double* resid, long long* iters, long long* inner) {
+  return guard([&] {
+    require(A && M && o && resid, "bad arguments");
+    if (!M->s.ulv) throw std::runtime_error("preconditioner not factorized");
+    if (mxg_gmres_batch(A->s.dev, M->s.ulv, o, nrhs, d_B, ldb, d_X, ldx, resid, iters, inner) != MXG_OK)
+      throw std::runtime_error(std::string("device layer: ") + mxg_last_error());
+  });
+}
+
 int mxc_dense_matvec(mxc_problem_t p, mxg_kernel_t k, const double* d_x, double* d_y) {
   return guard([&] {
     require(p, "null problem");

@@ -85,6 +85,14 @@ class Problem:
         ctx.adopt(self, Context.PROBLEMS)
 
     def dense_matvec(self, kernel: str, diag: float, alpha: float, x: DeviceArray, y: DeviceArray) -> None:
+        """y = K x with the matrix-free FP64 dense product; x and y may hold
+        several columns (n x nrhs)."""
+        if len(x.shape) > 1:
+            assert x.shape == y.shape and x.shape[0] == self.n
+            bodies = N.h2().mxc_problem_device_bodies(self.handle)
+            N.check_gpu(N.gpu().mxg_dense_matvec_batch(self.ctx.handle, kernel_desc(kernel, diag, alpha), bodies, self.n,
+                                                       x.ptr, self.n, y.ptr, self.n, x.shape[1]))
+            return
         N.check_h2(N.h2().mxc_dense_matvec(self.handle, kernel_desc(kernel, diag, alpha), x.ptr, y.ptr))
 
     def close(self):
@@ -174,6 +182,23 @@ class H2:
         N.check_h2(N.h2().mxc_solver_gmres(self.handle, M.handle, C.byref(o), b.ptr, x.ptr, resid.ctypes.data,
                                            C.byref(it), C.byref(inner)))
         return resid[: it.value + 1], it.value, inner.value
+
+    def gmres_batch(self, M: "H2", B: DeviceArray, X: DeviceArray, tol: float, restart: int = 10, max_cycles: int = 50,
+                    mode: int = N.GMRES_REFERENCE, inner_precision: int = N.MXG_FP64, inner: "H2" = None):
+        """GMRES for the columns of B (n x nrhs) in lockstep, solutions into X
+        (initial guesses on entry). Returns the residual histories (nrhs x
+        (max_cycles + 1)), cycles and inner iterations per right-hand side."""
+        n = self.problem.n
+        nrhs = B.shape[1] if len(B.shape) > 1 else 1
+        assert X.shape == B.shape and B.shape[0] == n
+        inner_dev = N.h2().mxc_solver_device(inner.handle) if inner is not None else None
+        o = N.MxgGmresOpts(tol, restart, max_cycles, mode, inner_precision, inner_dev)
+        resid = np.zeros((nrhs, max_cycles + 1))
+        it = np.zeros(nrhs, np.int64)
+        inn = np.zeros(nrhs, np.int64)
+        N.check_h2(N.h2().mxc_solver_gmres_batch(self.handle, M.handle, C.byref(o), nrhs, B.ptr, n, X.ptr, n,
+                                                 resid.ctypes.data, it.ctypes.data, inn.ctypes.data))
+        return [resid[q, : it[q] + 1] for q in range(nrhs)], it, inn
 
     def close(self):
         if self.handle and self.problem.ctx.handle:

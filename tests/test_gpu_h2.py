@@ -251,3 +251,41 @@ def test_gmres_with_looser_inner_operator(setup, operators, preconditioners):
     dx3 = ctx.array(dx.download())
     r3, it3, _ = A.gmres(M, db, dx3, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2)
     assert it3 == 0 and r3[0] < 1e-11
+
+
+def test_batched_right_hand_sides_match_single(setup, operators, preconditioners):
+    """Matvec, substitution and GMRES over a batch of 5 right-hand sides (one
+    launch of 4 and one of 1) give every column bit for bit what a solve of
+    that column alone gives."""
+    A, _ = operators
+    M, _ = preconditioners
+    ctx = setup["ctx"]
+    nq = 5
+    X = np.stack([O.uniform_vec(NPTS, -1, 1, 20 + q) for q in range(nq)], axis=1)
+    devA = N.h2().mxc_solver_device(A.handle)
+    for prec in (N.MXG_FP64, N.MXG_FP32):
+        dX, dY = ctx.array(X), ctx.empty((NPTS, nq))
+        N.check_gpu(N.gpu().mxg_h2_matvec(devA, dX.ptr, dY.ptr, nq, NPTS, prec))
+        Y = dY.download()
+        for q in range(nq):
+            assert np.array_equal(Y[:, q], gpu_matvec(ctx, A, X[:, q], prec)), (prec, q)
+    for fprec in (N.MXG_FP64, N.MXG_BF16):
+        M.factorize(fprec)
+        ulv = N.h2().mxc_solver_ulv(M.handle)
+        dZ = ctx.array(X)
+        N.check_gpu(N.gpu().mxg_ulv_solve(ulv, dZ.ptr, nq, NPTS))
+        Z = dZ.download()
+        for q in range(nq):
+            d = ctx.array(X[:, q])
+            M.precondition(d)
+            assert np.array_equal(Z[:, q], d.download()), (fprec, q)
+    M.factorize(N.MXG_FP64)
+    dB, dXb = ctx.array(X), ctx.zeros((NPTS, nq))
+    res, its, inn = A.gmres_batch(M, dB, dXb, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2,
+                                  inner_precision=N.MXG_FP32)
+    Xb = dXb.download()
+    for q in range(nq):
+        db, dx = ctx.array(X[:, q]), ctx.zeros(NPTS)
+        r1, i1, n1 = A.gmres(M, db, dx, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2, inner_precision=N.MXG_FP32)
+        assert res[q][-1] < 1e-11 and (its[q], inn[q]) == (i1, n1)
+        assert np.array_equal(Xb[:, q], dx.download()), q
