@@ -10,9 +10,12 @@
 // lanes run down the rows (16-byte vector loads of a column when the block
 // is aligned) and the four waves take every fourth column; short columns
 // share a wave. Each loaded column is applied to NR right-hand sides, so a
-// block is streamed from HBM once per batch. The partial rows are summed
-// through LDS in a fixed order, and the per-box sums read the terms back in
-// CSR order: every result is deterministic and does not depend on NR.
+// block is streamed from HBM once per batch; the NR values of a column sit
+// side by side in LDS and reach every lane in one broadcast read. Each row
+// is a chain of fused multiply-adds down the wave's columns in order, the
+// four waves' partial rows are summed through LDS in a fixed order, and the
+// per-box sums read the terms back in CSR order: every result is
+// deterministic and does not depend on NR.
 //
 // Blocks are stored in the compute type T (double or float) or, for the
 // FP32 substitution of a preconditioner, in bfloat16 (the upper 16 bits of
@@ -26,24 +29,43 @@ template <class T, int V> struct Vec { typedef T type __attribute__((ext_vector_
 template <> struct Vec<float, 1> { typedef float type; };
 template <> struct Vec<double, 1> { typedef double type; };
 
-// V elements of storage type S at p (16-byte aligned when V > 1) as T.
+// V elements of storage type S as loaded from memory (16 bytes when V > 1),
+// and their conversion to V elements of T: bfloat16 is the upper half of the
+// FP32 pattern. Loads stay packed until their column is used, so that more of
+// them fit in flight.
+template <class T, class S, int V> struct Raw { typedef typename Vec<T, V>::type type; };
+template <int V> struct Raw<float, bf16_t, V> { typedef unsigned short type __attribute__((ext_vector_type(V))); };
+template <> struct Raw<float, bf16_t, 1> { typedef unsigned short type; };
+
 template <class T, class S, int V>
-__device__ inline typename Vec<T, V>::type load_vec(const S* p) {
+__device__ inline typename Raw<T, S, V>::type load_raw(const S* p) {
+  typedef typename Raw<T, S, V>::type R;
+  return *reinterpret_cast<const R*>(p);
+}
+
+template <class T, class S, int V>
+__device__ inline typename Vec<T, V>::type cvt(const typename Raw<T, S, V>::type& u) {
   if constexpr (std::is_same<T, S>::value) {
-    return *reinterpret_cast<const typename Vec<T, V>::type*>(p);
+    return u;
   } else {
     static_assert(std::is_same<T, float>::value && std::is_same<S, bf16_t>::value, "bf16 storage needs float compute");
-    typename Vec<T, V>::type out;
     if constexpr (V == 1) {
-      out = __uint_as_float((unsigned)p[0] << 16);
+      return __uint_as_float((unsigned)u << 16);
     } else {
-      typedef unsigned short U __attribute__((ext_vector_type(V)));
-      const U u = *reinterpret_cast<const U*>(p);
+      typename Vec<T, V>::type out;
 #pragma unroll
       for (int e = 0; e < V; e++) out[e] = __uint_as_float((unsigned)u[e] << 16);
+      return out;
     }
-    return out;
   }
+}
+
+// LDS of a block product: the staged right-hand-side values and the cross-wave
+// row sums, RED_BYTES each.
+constexpr int RED_BYTES = 16384;
+template <class T>
+constexpr int lds_elems() {
+  return RED_BYTES / (int)sizeof(T);
 }
 
 // row groups of one pass of a workgroup (64 lanes x V rows x QN groups):
@@ -52,11 +74,37 @@ template <int V, int NR>
 constexpr int qn() {
   return V == 1 ? 4 : V == 2 ? 2 : (V == 4 && NR == 1) ? 2 : 1;
 }
-// columns in flight per wave (the same for every NR, so that a row's sum is
-// formed in the same order whatever the batch width)
-template <int NR>
+// columns whose loads a wave keeps in flight; the sums run down the columns in
+// order whatever this is, so it may depend on NR without changing a result
+template <int V, int NR>
 constexpr int unroll() {
   return 4;
+}
+
+// Stage columns [cb, cb + cn) of the NR right-hand sides as vs[c NR + k], so
+// that a column's NR values are one broadcast LDS read.
+template <class T, int NR, class VF>
+__device__ inline void stage_v(T* vs, int cb, int cn, VF vfun) {
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NR; k++)
+    for (int c = threadIdx.x; c < cn; c += 256) vs[c * NR + k] = vfun(cb + c, k);
+  __syncthreads();
+}
+
+template <class T, int NR>
+__device__ inline typename Vec<T, NR>::type lds_row(const T* p) {
+  return *reinterpret_cast<const typename Vec<T, NR>::type*>(p);
+}
+
+template <class T, int NR, class VT, class WT>
+__device__ inline void fma_col(VT* acc, const VT& a, const WT& w) {
+  if constexpr (NR == 1) {
+    acc[0] += a * w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < NR; k++) acc[k] += a * w[k];
+  }
 }
 
 // Partial rows of B[r0 : r0 + 64 V QN, :] v_k for this wave's columns (c =
@@ -64,11 +112,15 @@ constexpr int unroll() {
 // handed to emit(r, k, value) by thread r (and r + 256, ...).
 template <class T, class S, int V, int NR, class VF, class EF>
 __device__ inline void wg_gemv_pass(const S* __restrict__ B, long long ld, int n, int m, int r0, VF vfun, EF emit,
-                                    T* red) {
+                                    T* red, T* vs) {
   typedef typename Vec<T, V>::type VT;
+  typedef typename Vec<T, NR>::type WT;
+  typedef typename Raw<T, S, V>::type RT;
   constexpr int QN = qn<V, NR>();
-  constexpr int U = unroll<NR>();
+  constexpr int U = unroll<V, NR>();
   constexpr int ROWS = 64 * V * QN;
+  constexpr int CW = lds_elems<T>() / NR;
+  static_assert(CW % 256 == 0, "column chunks must keep every wave's column sequence");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   VT acc[QN][NR];
 #pragma unroll
@@ -76,56 +128,57 @@ __device__ inline void wg_gemv_pass(const S* __restrict__ B, long long ld, int n
 #pragma unroll
     for (int k = 0; k < NR; k++) acc[q][k] = VT(0);
   const int nrows = min(ROWS, n - r0);
-  int c = wave;
-  for (; c + 4 * (U - 1) < m; c += 4 * U) {
-    T vv[U][NR];
-    VT a[U][QN];
+  for (int cb = 0; cb < m; cb += CW) {
+    const int cn = min(CW, m - cb);
+    stage_v<T, NR>(vs, cb, cn, vfun);
+    const S* Bc = B + (long long)cb * ld + r0;
+    int c = wave;
+    for (; c + 4 * (U - 1) < cn; c += 4 * U) {
+      RT a[U][QN];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+      for (int u = 0; u < U; u++) {
+        const S* col = Bc + (long long)(c + 4 * u) * ld;
 #pragma unroll
-      for (int k = 0; k < NR; k++) vv[u][k] = vfun(c + 4 * u, k);
-      const S* col = B + (long long)(c + 4 * u) * ld + r0;
+        for (int q = 0; q < QN; q++) {
+          const int r = (q * 64 + lane) * V;
+          a[u][q] = (r < nrows) ? load_raw<T, S, V>(col + r) : RT(0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const WT w = lds_row<T, NR>(vs + (c + 4 * u) * NR);
+#pragma unroll
+        for (int q = 0; q < QN; q++) fma_col<T, NR>(acc[q], cvt<T, S, V>(a[u][q]), w);
+      }
+    }
+    for (; c < cn; c += 4) {
+      const S* col = Bc + (long long)c * ld;
+      const WT w = lds_row<T, NR>(vs + c * NR);
 #pragma unroll
       for (int q = 0; q < QN; q++) {
         const int r = (q * 64 + lane) * V;
-        a[u][q] = (r < nrows) ? load_vec<T, S, V>(col + r) : VT(0);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < QN; q++)
-#pragma unroll
-      for (int k = 0; k < NR; k++) {
-        if constexpr (U == 4)
-          acc[q][k] += (a[0][q] * vv[0][k] + a[1][q] * vv[1][k]) + (a[2][q] * vv[2][k] + a[3][q] * vv[3][k]);
-        else
-          acc[q][k] += a[0][q] * vv[0][k] + a[1][q] * vv[1][k];
-      }
-  }
-  for (; c < m; c += 4) {
-    const S* col0 = B + (long long)c * ld + r0;
-    T v0[NR];
-#pragma unroll
-    for (int k = 0; k < NR; k++) v0[k] = vfun(c, k);
-#pragma unroll
-    for (int q = 0; q < QN; q++) {
-      const int r = (q * 64 + lane) * V;
-      if (r < nrows) {
-        const VT a0 = load_vec<T, S, V>(col0 + r);
-#pragma unroll
-        for (int k = 0; k < NR; k++) acc[q][k] += a0 * v0[k];
+        if (r < nrows) fma_col<T, NR>(acc[q], cvt<T, S, V>(load_raw<T, S, V>(col + r)), w);
       }
     }
   }
+  constexpr int KR = (lds_elems<T>() / (4 * ROWS)) < NR ? (lds_elems<T>() / (4 * ROWS)) : NR;
+  static_assert(KR >= 1 && NR % KR == 0, "reduction rounds");
 #pragma unroll
-  for (int k = 0; k < NR; k++) {
+  for (int k0 = 0; k0 < NR; k0 += KR) {
 #pragma unroll
-    for (int q = 0; q < QN; q++) {
-      const int r = (q * 64 + lane) * V;
-      if (r < nrows) *reinterpret_cast<VT*>(red + wave * ROWS + r) = acc[q][k];
-    }
+    for (int kk = 0; kk < KR; kk++)
+#pragma unroll
+      for (int q = 0; q < QN; q++) {
+        const int r = (q * 64 + lane) * V;
+        if (r < nrows) *reinterpret_cast<VT*>(red + (kk * 4 + wave) * ROWS + r) = acc[q][k0 + kk];
+      }
     __syncthreads();
-    for (int r = threadIdx.x; r < nrows; r += 256)
-      emit(r0 + r, k, red[r] + red[ROWS + r] + red[2 * ROWS + r] + red[3 * ROWS + r]);
+#pragma unroll
+    for (int kk = 0; kk < KR; kk++) {
+      const T* rk = red + kk * 4 * ROWS;
+      for (int r = threadIdx.x; r < nrows; r += 256)
+        emit(r0 + r, k0 + kk, rk[r] + rk[ROWS + r] + rk[2 * ROWS + r] + rk[3 * ROWS + r]);
+    }
     __syncthreads();
   }
 }
@@ -134,9 +187,14 @@ __device__ inline void wg_gemv_pass(const S* __restrict__ B, long long ld, int n
 // and each wave works on 64 / lpc columns at once, so no lane idles for want
 // of rows. The 4 * 64 / lpc partial columns are summed through LDS.
 template <class T, class S, int V, int NR, class VF, class EF>
-__device__ inline void wg_gemv_narrow(const S* __restrict__ B, long long ld, int n, int m, VF vfun, EF emit, T* red) {
+__device__ inline void wg_gemv_narrow(const S* __restrict__ B, long long ld, int n, int m, VF vfun, EF emit, T* red,
+                                      T* vs) {
   typedef typename Vec<T, V>::type VT;
-  constexpr int U = unroll<NR>();
+  typedef typename Vec<T, NR>::type WT;
+  typedef typename Raw<T, S, V>::type RT;
+  constexpr int U = unroll<V, NR>();
+  constexpr int CW = lds_elems<T>() / NR;
+  static_assert(CW % 256 == 0, "column chunks must keep every wave's column sequence");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nv = (n + V - 1) / V;
   int lpc = 1;
@@ -148,59 +206,53 @@ __device__ inline void wg_gemv_narrow(const S* __restrict__ B, long long ld, int
   VT acc[NR];
 #pragma unroll
   for (int k = 0; k < NR; k++) acc[k] = VT(0);
-  int c = wave * G + g;
-  for (; c + (U - 1) * stride < m; c += U * stride) {
-    T vv[U][NR];
-    VT a[U];
+  for (int cb = 0; cb < m; cb += CW) {
+    const int cn = min(CW, m - cb);
+    stage_v<T, NR>(vs, cb, cn, vfun);
+    const S* Bc = B + (long long)cb * ld + r;
+    int c = wave * G + g;
+    for (; c + (U - 1) * stride < cn; c += U * stride) {
+      RT a[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
+      for (int u = 0; u < U; u++) a[u] = active ? load_raw<T, S, V>(Bc + (long long)(c + u * stride) * ld) : RT(0);
 #pragma unroll
-      for (int k = 0; k < NR; k++) vv[u][k] = vfun(c + u * stride, k);
-      a[u] = active ? load_vec<T, S, V>(B + (long long)(c + u * stride) * ld + r) : VT(0);
+      for (int u = 0; u < U; u++) fma_col<T, NR>(acc, cvt<T, S, V>(a[u]), lds_row<T, NR>(vs + (c + u * stride) * NR));
     }
-#pragma unroll
-    for (int k = 0; k < NR; k++) {
-      if constexpr (U == 4)
-        acc[k] += (a[0] * vv[0][k] + a[1] * vv[1][k]) + (a[2] * vv[2][k] + a[3] * vv[3][k]);
-      else
-        acc[k] += a[0] * vv[0][k] + a[1] * vv[1][k];
-    }
-  }
-  for (; c < m; c += stride) {
-    if (active) {
-      const VT a0 = load_vec<T, S, V>(B + (long long)c * ld + r);
-#pragma unroll
-      for (int k = 0; k < NR; k++) acc[k] += a0 * vfun(c, k);
-    }
+    for (; c < cn; c += stride)
+      if (active)
+        fma_col<T, NR>(acc, cvt<T, S, V>(load_raw<T, S, V>(Bc + (long long)c * ld)), lds_row<T, NR>(vs + c * NR));
   }
   const int W = lpc * V;  // row stride of the partial columns in LDS
   const int slots = 4 * G;
+  constexpr int PK = 256 * V;  // LDS elements of one right-hand side's partial columns
+  constexpr int KR = (lds_elems<T>() / PK) < NR ? (lds_elems<T>() / PK) : NR;
+  static_assert(KR >= 1 && NR % KR == 0, "reduction rounds");
 #pragma unroll
-  for (int k = 0; k < NR; k++) {
-    *reinterpret_cast<VT*>(red + (wave * G + g) * W + r) = active ? acc[k] : VT(0);
+  for (int k0 = 0; k0 < NR; k0 += KR) {
+#pragma unroll
+    for (int kk = 0; kk < KR; kk++)
+      *reinterpret_cast<VT*>(red + kk * PK + (wave * G + g) * W + r) = active ? acc[k0 + kk] : VT(0);
     __syncthreads();
-    for (int rr = threadIdx.x; rr < n; rr += 256) {
-      T sum = T(0);
-      for (int s = 0; s < slots; s++) sum += red[s * W + rr];
-      emit(rr, k, sum);
+#pragma unroll
+    for (int kk = 0; kk < KR; kk++) {
+      const T* rk = red + kk * PK;
+      for (int rr = threadIdx.x; rr < n; rr += 256) {
+        T sum = T(0);
+        for (int s = 0; s < slots; s++) sum += rk[s * W + rr];
+        emit(rr, k0 + kk, sum);
+      }
     }
     __syncthreads();
   }
 }
 
-// LDS elements the block product needs with V rows per lane
-template <int V>
-constexpr int red_elems() {
-  return 4 * 64 * V * qn<V, 1>();
-}
-
 template <class T, class S, int V, int NR, class VF, class EF>
-__device__ inline void wg_gemv(const S* __restrict__ B, long long ld, int n, int m, VF vfun, EF emit, T* red) {
+__device__ inline void wg_gemv(const S* __restrict__ B, long long ld, int n, int m, VF vfun, EF emit, T* red, T* vs) {
   if (n <= 64 * V) {
-    wg_gemv_narrow<T, S, V, NR>(B, ld, n, m, vfun, emit, red);
+    wg_gemv_narrow<T, S, V, NR>(B, ld, n, m, vfun, emit, red, vs);
     return;
   }
-  for (int r0 = 0; r0 < n; r0 += 64 * V * qn<V, NR>()) wg_gemv_pass<T, S, V, NR>(B, ld, n, m, r0, vfun, emit, red);
+  for (int r0 = 0; r0 < n; r0 += 64 * V * qn<V, NR>()) wg_gemv_pass<T, S, V, NR>(B, ld, n, m, r0, vfun, emit, red, vs);
 }
 
 template <class S>
@@ -208,11 +260,103 @@ constexpr int vwidth() {
   return 16 / (int)sizeof(S);
 }
 
-// Right-hand sides c0 .. c0 + NR - 1 of every job.
+// Orders this wave's LDS accesses: those before complete before those after.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A whole small block by one wave (n <= 64 V): a column takes lpc =
+// 2^ceil(log2(n / V)) lanes, the wave's G = 64 / lpc lane groups take every
+// G-th column in order, and the G partial columns are summed by a butterfly
+// over the lane groups. No workgroup barrier: the four waves of a workgroup
+// work on four independent blocks, each staging its right-hand-side values
+// in its own quarter of vs.
+template <class T, class S, int V, int NR>
+__device__ inline void wave_gemv(const PairJob& t, int c0, T* __restrict__ vw) {
+  typedef typename Vec<T, V>::type VT;
+  typedef typename Vec<T, NR>::type WT;
+  typedef typename Raw<T, S, V>::type RT;
+  constexpr int U = unroll<V, NR>();
+  constexpr int CW = lds_elems<T>() / 4 / NR;  // columns per staged chunk
+  static_assert(CW % 64 == 0, "column chunks must keep every lane group's column sequence");
+  const S* B = static_cast<const S*>(t.B);
+  const T* vb = static_cast<const T*>(t.v) + (long long)c0 * t.ldv;
+  T* ob = static_cast<T*>(t.out) + (long long)c0 * t.ldo;
+  const int lane = threadIdx.x & 63;
+  const int n = t.n, m = t.m;
+  const int nv = (n + V - 1) / V;
+  int lpc = 1;
+  while (lpc < nv) lpc <<= 1;
+  const int G = 64 / lpc, g = lane / lpc, r = (lane % lpc) * V;
+  const bool active = r < n;
+  VT acc[NR];
+#pragma unroll
+  for (int k = 0; k < NR; k++) acc[k] = VT(0);
+  for (int cb = 0; cb < m; cb += CW) {
+    const int cn = min(CW, m - cb);
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < NR; k++)
+      for (int c = lane; c < cn; c += 64) vw[c * NR + k] = vb[cb + c + (long long)k * t.ldv];
+    wave_lds_sync();
+    // lanes past the column's end load its first V rows; their sums are dropped
+    const S* Bc = B + (long long)cb * t.ld + (active ? r : 0);
+    int c = g;
+    for (; c + (U - 1) * G < cn; c += U * G) {
+      RT a[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) a[u] = load_raw<T, S, V>(Bc + (long long)(c + u * G) * t.ld);
+#pragma unroll
+      for (int u = 0; u < U; u++) fma_col<T, NR>(acc, cvt<T, S, V>(a[u]), lds_row<T, NR>(vw + (c + u * G) * NR));
+    }
+    for (; c < cn; c += G)
+      fma_col<T, NR>(acc, cvt<T, S, V>(load_raw<T, S, V>(Bc + (long long)c * t.ld)), lds_row<T, NR>(vw + c * NR));
+  }
+  for (int sh = lpc; sh < 64; sh <<= 1)
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      if constexpr (V == 1) {
+        acc[k] += __shfl_xor(acc[k], sh);
+      } else {
+#pragma unroll
+        for (int e = 0; e < V; e++) acc[k][e] += __shfl_xor(acc[k][e], sh);
+      }
+    }
+  if (g == 0 && active)
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      T* o = ob + (long long)k * t.ldo + r;
+      if constexpr (V == 1) {
+        o[0] = acc[k];
+      } else {
+#pragma unroll
+        for (int e = 0; e < V; e++) o[e] = acc[k][e];
+      }
+    }
+}
+
+// Right-hand sides c0 .. c0 + NR - 1 of every job. Workgroups below nbig take
+// one workgroup-sized job each; the others take four wave-sized jobs.
 template <class T, class S, int NR>
-__global__ void __launch_bounds__(256) k_pair_gemv(const PairJob* __restrict__ jobs, int c0) {
+__global__ void __launch_bounds__(256) k_pair_gemv(const PairJob* __restrict__ jobs, long long nbig, long long count,
+                                                   int c0) {
   constexpr int VW = vwidth<S>();
-  __shared__ __align__(16) T red[red_elems<VW>()];
+  __shared__ __align__(16) T red[lds_elems<T>()];
+  __shared__ __align__(16) T vs[lds_elems<T>()];
+  if ((long long)blockIdx.x >= nbig) {
+    const int wave = threadIdx.x >> 6;
+    const long long j = nbig + ((long long)blockIdx.x - nbig) * 4 + wave;
+    if (j >= count) return;
+    const PairJob t = jobs[j];
+    T* vw = vs + wave * (lds_elems<T>() / 4);
+    if (t.vec)
+      wave_gemv<T, S, VW, NR>(t, c0, vw);
+    else
+      wave_gemv<T, S, 1, NR>(t, c0, vw);
+    return;
+  }
   const PairJob t = jobs[blockIdx.x];
   const S* B = static_cast<const S*>(t.B);
   const T* vb = static_cast<const T*>(t.v) + (long long)c0 * t.ldv;
@@ -225,9 +369,9 @@ __global__ void __launch_bounds__(256) k_pair_gemv(const PairJob* __restrict__ j
   auto vf = [&](int c, int k) { return vb[c + (long long)k * t.ldv]; };
   auto ef = [&](int r, int k, T s) { ob[r + (long long)k * t.ldo] = s; };
   if (t.vec)
-    wg_gemv<T, S, VW, NR>(B, t.ld, t.n, t.m, vf, ef, red);
+    wg_gemv<T, S, VW, NR>(B, t.ld, t.n, t.m, vf, ef, red, vs);
   else
-    wg_gemv<T, S, 1, NR>(B, t.ld, t.n, t.m, vf, ef, red);
+    wg_gemv<T, S, 1, NR>(B, t.ld, t.n, t.m, vf, ef, red, vs);
 }
 
 // Strides (elements) between right-hand sides of the vectors of a per-box sum.
@@ -246,7 +390,8 @@ __global__ void __launch_bounds__(256) k_comb_down(const int* __restrict__ dims,
                                                    const long long* __restrict__ toffN, const T* __restrict__ Q,
                                                    const T* __restrict__ w_in, const T* __restrict__ tF,
                                                    const T* __restrict__ tN, T* __restrict__ y, Strides ld, int kmax) {
-  __shared__ __align__(16) T red[red_elems<V>()];
+  __shared__ __align__(16) T red[lds_elems<T>()];
+  __shared__ __align__(16) T vs[lds_elems<T>()];
   extern __shared__ __align__(16) unsigned char dsm[];
   T* w = reinterpret_cast<T*>(dsm);  // kmax x NR
   const int i = blockIdx.x;
@@ -269,7 +414,7 @@ __global__ void __launch_bounds__(256) k_comb_down(const int* __restrict__ dims,
       for (int r = threadIdx.x; r < M; r += 256) emit(r, q, T(0));
     return;
   }
-  wg_gemv<T, T, V, NR>(Q + offQ[i], M, M, k, [&](int c, int q) { return w[c + q * kmax]; }, emit, red);
+  wg_gemv<T, T, V, NR>(Q + offQ[i], M, M, k, [&](int c, int q) { return w[c + q * kmax]; }, emit, red, vs);
 }
 
 // Forward substitution, skeleton part: z_i = x_i[:s] - sum_near t_p.
@@ -298,7 +443,8 @@ __global__ void __launch_bounds__(256) k_comb_bwd(const int* __restrict__ dims, 
                                                   const long long* __restrict__ toff, const S* __restrict__ U,
                                                   const T* __restrict__ x, const T* __restrict__ w,
                                                   const T* __restrict__ t, T* __restrict__ y, Strides ld, int mmax) {
-  __shared__ __align__(16) T red[red_elems<V>()];
+  __shared__ __align__(16) T red[lds_elems<T>()];
+  __shared__ __align__(16) T vs[lds_elems<T>()];
   extern __shared__ __align__(16) unsigned char dsm[];
   T* xv = reinterpret_cast<T*>(dsm);  // mmax x NR
   const int i = blockIdx.x;
@@ -317,7 +463,7 @@ __global__ void __launch_bounds__(256) k_comb_bwd(const int* __restrict__ dims, 
   __syncthreads();
   T* yi = y + offX[i];
   wg_gemv<T, S, V, NR>(U + offQ[i], M, M, M, [&](int c, int q) { return xv[c + q * mmax]; },
-                       [&](int r, int q, T v) { yi[r + q * ld.d] = v; }, red);
+                       [&](int r, int q, T v) { yi[r + q * ld.d] = v; }, red, vs);
 }
 
 __global__ void k_to_bf16(bf16_t* __restrict__ dst, const float* __restrict__ src, size_t n) {
@@ -326,6 +472,13 @@ __global__ void k_to_bf16(bf16_t* __restrict__ dst, const float* __restrict__ sr
     dst[e] = (bf16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
   }
 }
+
+// dynamic LDS of a per-box sum (its box vector for every right-hand side of
+// the launch), on top of the 2 RED_BYTES of the block product: 64 KiB in all
+constexpr size_t DYN_LDS_BYTES = 65536 - 2 * RED_BYTES;
+
+// blocks up to this size (and with n <= 64 rows per vector lane) go to one wave
+constexpr long long WAVE_JOB_BYTES = 65536;
 
 // largest supported batch width not above nr
 inline int batch_width(int nr) { return nr >= 8 ? 8 : nr >= 4 ? 4 : nr >= 2 ? 2 : 1; }
@@ -343,7 +496,8 @@ void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& t
     MXG_REQUIRE(t.n == 0 || t.m == 0 || t.b + (long long)(t.m - 1) * t.ld + t.n <= nB, "pair task outside its matrix");
     const S* b = B + t.b;
     const int vec = ((reinterpret_cast<uintptr_t>(b) % 16) == 0 && t.ld % W == 0 && t.n % W == 0) ? 1 : 0;
-    jobs.push_back(PairJob{b, v + t.v, out + t.out, ldv, ldo, t.n, t.m, t.ld, vec});
+    const int wave = (t.n <= 64 * (vec ? W : 1) && (long long)t.n * t.m * (long long)sizeof(S) <= WAVE_JOB_BYTES) ? 1 : 0;
+    jobs.push_back(PairJob{b, v + t.v, out + t.out, ldv, ldo, t.n, t.m, t.ld, vec, wave});
   }
 }
 template void pair_jobs_append<float, float>(std::vector<PairJob>&, const std::vector<PairTask>&, const float*, long long,
@@ -356,22 +510,31 @@ template void pair_jobs_append<float, bf16_t>(std::vector<PairJob>&, const std::
 
 void pair_list_upload(PairList& pl, const std::vector<PairJob>& jobs, hipStream_t s) {
   MXG_REQUIRE(jobs.size() < (1ull << 31), "pair list too long");
-  pl.count = (long long)jobs.size();
-  if (pl.count) pl.jobs.upload(jobs, s);
+  std::vector<PairJob> sorted;
+  sorted.reserve(jobs.size());
+  for (const PairJob& j : jobs)
+    if (!j.wave) sorted.push_back(j);
+  pl.nbig = (long long)sorted.size();
+  for (const PairJob& j : jobs)
+    if (j.wave) sorted.push_back(j);
+  pl.count = (long long)sorted.size();
+  if (pl.count) pl.jobs.upload(sorted, s);
 }
 
 template <class T, class S>
 void pair_gemv(const PairList& pl, int nrhs, hipStream_t s) {
   if (pl.count == 0) return;
   MXG_REQUIRE(nrhs >= 1 && nrhs <= MAX_RHS, "right-hand sides out of range");
+  const long long groups = pl.nbig + (pl.count - pl.nbig + 3) / 4;
+  MXG_REQUIRE(groups < (1ll << 31), "pair list too long");
   for (int c0 = 0; c0 < nrhs;) {
     const int nr = batch_width(nrhs - c0);
-    const dim3 g((unsigned)pl.count), b(256);
+    const dim3 g((unsigned)groups), b(256);
     switch (nr) {
-      case 8: hipLaunchKernelGGL((k_pair_gemv<T, S, 8>), g, b, 0, s, pl.jobs.p, c0); break;
-      case 4: hipLaunchKernelGGL((k_pair_gemv<T, S, 4>), g, b, 0, s, pl.jobs.p, c0); break;
-      case 2: hipLaunchKernelGGL((k_pair_gemv<T, S, 2>), g, b, 0, s, pl.jobs.p, c0); break;
-      default: hipLaunchKernelGGL((k_pair_gemv<T, S, 1>), g, b, 0, s, pl.jobs.p, c0); break;
+      case 8: hipLaunchKernelGGL((k_pair_gemv<T, S, 8>), g, b, 0, s, pl.jobs.p, pl.nbig, pl.count, c0); break;
+      case 4: hipLaunchKernelGGL((k_pair_gemv<T, S, 4>), g, b, 0, s, pl.jobs.p, pl.nbig, pl.count, c0); break;
+      case 2: hipLaunchKernelGGL((k_pair_gemv<T, S, 2>), g, b, 0, s, pl.jobs.p, pl.nbig, pl.count, c0); break;
+      default: hipLaunchKernelGGL((k_pair_gemv<T, S, 1>), g, b, 0, s, pl.jobs.p, pl.nbig, pl.count, c0); break;
     }
     MXG_HIP(hipGetLastError());
     c0 += nr;
@@ -386,10 +549,11 @@ void comb_down(const Level& Lv, bool vec, const long long* d_toffF, const long l
                long long ldw, const T* tF, const T* tN, long long ldt, T* y, long long ldy, int nrhs, hipStream_t s) {
   if (Lv.nb == 0) return;
   const int kmax = (int)std::max<long long>(Lv.maxK, 1);
+  const int fit = (int)(DYN_LDS_BYTES / ((size_t)kmax * sizeof(T)));
+  MXG_REQUIRE(fit >= 1, "rank too large for the downward pass");
   for (int c0 = 0; c0 < nrhs;) {
-    const int nr = batch_width(nrhs - c0);
+    const int nr = batch_width(std::min(nrhs - c0, fit));
     const size_t shm = (size_t)kmax * nr * sizeof(T);
-    MXG_REQUIRE(shm <= 64 * 1024, "rank too large for the downward pass");
     const Strides ld{ldw, ldt, 0, ldy};
     const T* wi = w_in + c0 * ldw;
     const T* f = tF + c0 * ldt;
@@ -442,10 +606,11 @@ void comb_bwd(const Level& Lv, bool vec, const long long* d_toff, const S* U, co
               long long ldw, const T* t, long long ldt, T* y, long long ldy, int nrhs, hipStream_t s) {
   if (Lv.nb == 0) return;
   const int mmax = (int)std::max<long long>(Lv.maxM, 1);
+  const int fit = (int)(DYN_LDS_BYTES / ((size_t)mmax * sizeof(T)));
+  MXG_REQUIRE(fit >= 1, "box too large for the backward pass");
   for (int c0 = 0; c0 < nrhs;) {
-    const int nr = batch_width(nrhs - c0);
+    const int nr = batch_width(std::min(nrhs - c0, fit));
     const size_t shm = (size_t)mmax * nr * sizeof(T);
-    MXG_REQUIRE(shm <= 64 * 1024, "box too large for the backward pass");
     const Strides ld{ldx, ldw, ldt, ldy};
     const T* xi = x + c0 * ldx;
     const T* wi = w + c0 * ldw;

@@ -32,13 +32,17 @@ struct PairJob {
   const void* v;
   void* out;
   long long ldv, ldo;  // strides (elements) between right-hand sides of v and out
-  int n, m, ld, vec;
+  int n, m, ld;
+  int vec;   // 1: B 16-byte aligned with ld and n multiples of the vector width
+  int wave;  // 1: small enough for one wave (handled four to a workgroup)
 };
 // right-hand sides one launch of the solve-phase kernels carries at most
 constexpr int MAX_RHS = 8;
+// Jobs of one launch: the workgroup-sized ones first, then the wave-sized.
 struct PairList {
   DevBuf<PairJob> jobs;
   long long count = 0;
+  long long nbig = 0;
 };
 
 struct Level {
