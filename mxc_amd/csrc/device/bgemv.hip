@@ -81,6 +81,8 @@ constexpr int unroll() {
   return 4;
 }
 
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 // Stage columns [cb, cb + cn) of the NR right-hand sides as vs[c NR + k], so
 // that a column's NR values are one broadcast LDS read.
 template <class T, int NR, class VF>
@@ -121,7 +123,7 @@ __device__ inline void wg_gemv_pass(const S* __restrict__ B, long long ld, int n
   constexpr int ROWS = 64 * V * QN;
   constexpr int CW = lds_elems<T>() / NR;
   static_assert(CW % 256 == 0, "column chunks must keep every wave's column sequence");
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_id();
   VT acc[QN][NR];
 #pragma unroll
   for (int q = 0; q < QN; q++)
@@ -195,7 +197,7 @@ __device__ inline void wg_gemv_narrow(const S* __restrict__ B, long long ld, int
   constexpr int U = unroll<V, NR>();
   constexpr int CW = lds_elems<T>() / NR;
   static_assert(CW % 256 == 0, "column chunks must keep every wave's column sequence");
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_id();
   const int nv = (n + V - 1) / V;
   int lpc = 1;
   while (lpc < nv) lpc <<= 1;
@@ -257,7 +259,7 @@ __device__ inline void wg_gemv(const S* __restrict__ B, long long ld, int n, int
 
 template <class S>
 constexpr int vwidth() {
-  return 16 / (int)sizeof(S);
+  return std::is_same<S, bf16_t>::value ? 4 : 16 / (int)sizeof(S);
 }
 
 // Orders this wave's LDS accesses: those before complete before those after.
@@ -346,7 +348,7 @@ __global__ void __launch_bounds__(256) k_pair_gemv(const PairJob* __restrict__ j
   __shared__ __align__(16) T red[lds_elems<T>()];
   __shared__ __align__(16) T vs[lds_elems<T>()];
   if ((long long)blockIdx.x >= nbig) {
-    const int wave = threadIdx.x >> 6;
+    const int wave = wave_id();
     const long long j = nbig + ((long long)blockIdx.x - nbig) * 4 + wave;
     if (j >= count) return;
     const PairJob t = jobs[j];
