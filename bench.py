@@ -68,7 +68,8 @@ def measured_traffic(nr):
     try:
         text = open(path).read()
         rec = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
-        return float(rec["traffic_bytes"]) if int(rec["rhs"]) == nr else None
+        same = int(rec["rhs"]) == nr and rec.get("launch") == "leaf near products"
+        return float(rec["traffic_bytes"]) if same else None
     except (OSError, IndexError, KeyError, ValueError):
         return None
 
@@ -151,8 +152,8 @@ def main():
         N.check_gpu(N.gpu().mxg_profile_enable(ctx.handle, 0))
         kern = {}
         import ctypes as C
-        for cls, name in ((0, "matvec_leaf_fp64"), (1, "matvec_leaf_fp32"), (2, "ulv_fwd_leaf"), (3, "ulv_bwd_leaf"),
-                          (4, "matvec_upper")):
+        for cls, name in ((0, "matvec_near_fp64"), (1, "matvec_near_fp32"), (2, "ulv_fwd_leaf"), (3, "ulv_bwd_leaf"),
+                          (4, "matvec_upper"), (5, "matvec_far")):
             cnt, ms = C.c_longlong(), C.c_double()
             N.check_gpu(N.gpu().mxg_profile_read(ctx.handle, cls, C.byref(cnt), C.byref(ms)))
             kern[name] = {"launches": cnt.value, "total_ms": ms.value,
@@ -173,20 +174,16 @@ def main():
     fp64 = run_path(N.MXG_FP64, N.MXG_FP64, 0)
     mixed = run_path(N.MXG_BF16 if args.mixed_factor == "bf16" else N.MXG_FP32, N.MXG_FP32, 1)
 
-    # roofline of the dominant kernel of the mixed path: the FP32 product
-    # launch of the matvec for the batch's right-hand sides. Algorithmic bytes:
-    # every leaf near block and every coupling block of every level once, and
-    # per right-hand side the inputs (x, every level's skeleton weights z) read
-    # once and the outputs (the near sum of every leaf row, the far sum of every
-    # skeleton entry) written once.
+    # roofline of the dominant kernel of the mixed path: the FP32 launch of the
+    # leaf near products of the matvec for the batch's right-hand sides.
+    # Algorithmic bytes: every leaf near block once, and per right-hand side
+    # the input x read once and the near sum of every leaf row written once.
     Afp32 = A_in if A_in is not None else A
     h = N.h2().mxc_solver_device(Afp32.handle)
     near_elems = int(N.gpu().mxg_h2_level_size(h, L, 0))
-    coupling_elems = sum(int(N.gpu().mxg_h2_level_size(h, l, 3)) for l in range(L + 1))
-    skeleton = sum(int(Afp32.dims(l)[1].sum()) for l in range(L + 1))
     nr = min(bs, 8)  # right-hand sides per launch
-    bytes_prod_fp32 = 4 * (near_elems + coupling_elems + 2 * nr * (n + skeleton))
-    km = mixed["kernels"]["matvec_leaf_fp32"]
+    bytes_prod_fp32 = 4 * (near_elems + 2 * nr * n)
+    km = mixed["kernels"]["matvec_near_fp32"]
     achieved = bytes_prod_fp32 / (km["avg_ms"] * 1e-3) / 1e9 if km["avg_ms"] else None
     sizes = []
     hA = N.h2().mxc_solver_device(A.handle)
@@ -235,8 +232,7 @@ def main():
         "kernels_mixed": mixed["kernels"],
         "operator_levels": sizes,
         "roofline": {
-            "kernel": f"k_pair_gemv<float, float, {nr}>: the matvec's product launch (leaf near blocks + "
-                      "couplings of every level, FP32)",
+            "kernel": f"k_pair_gemv<float, float, {nr}>: the matvec's launch of the leaf near products (FP32)",
             "bound": "hbm",
             "achieved": round(achieved, 1) if achieved else None,
             "peak": HBM_PEAK_GBS,

@@ -169,17 +169,20 @@ int mxg_gmres_batch(mxg_h2_t A, mxg_ulv_t M, const mxg_gmres_opts_t* opts, long 
                     long long ldb, double* d_X, long long ldx, double* resid, long long* iters_out,
                     long long* inner_out);
 
-/* ---- kernel timing (HIP events on the context stream) ----
+/* ---- kernel timing (HIP events on the stream each launch runs on) ----
  * While enabled, every launch of the instrumented kernel classes is bracketed
  * by a pair of HIP events; mxg_profile_read synchronizes and returns the
- * number of launches and their summed device time since the last reset. */
+ * number of launches and their summed device time since the last reset. The
+ * leaf near products run on a second stream of the context, concurrently
+ * with the upward pass and the coupling products. */
 enum {
   MXG_PROF_MATVEC_LEAF_FP64 = 0, /* near-field product of the leaf level (one kernel launch) */
   MXG_PROF_MATVEC_LEAF_FP32 = 1,
   MXG_PROF_ULV_FWD_LEAF = 2,     /* forward substitution, leaf level (x = V y and z) */
   MXG_PROF_ULV_BWD_LEAF = 3,     /* backward substitution, leaf level */
-  MXG_PROF_MATVEC_UPPER = 4,     /* every other matvec launch: bases, couplings, per-box sums */
-  MXG_PROF_CLASSES = 5
+  MXG_PROF_MATVEC_UPPER = 4,     /* upward products and per-box sums of every level */
+  MXG_PROF_MATVEC_FAR = 5,       /* coupling products of every level (one launch) */
+  MXG_PROF_CLASSES = 6
 };
 int mxg_profile_enable(mxg_ctx_t ctx, int enable);
 int mxg_profile_reset(mxg_ctx_t ctx);

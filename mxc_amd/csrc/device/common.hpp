@@ -142,6 +142,10 @@ __device__ inline int find_task(const long long* prefix, int ntasks, long long x
 struct mxg_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  // a second stream for work that may overlap the context stream's (the
+  // leaf near products of a matvec), forked from and joined back into it
+  hipStream_t aux = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   // kernel timing: while enabled, instrumented launches are bracketed by HIP
   // events on the context stream (one start/stop pair per launch)
   bool profiling = false;
@@ -155,12 +159,14 @@ struct mxg_ctx_s {
 };
 
 namespace mxg {
-// Brackets one launch of kernel class `cls` with events when profiling.
+// Brackets one launch of kernel class `cls` with events when profiling, on
+// the context stream or on stream s.
 struct ProfScope {
   mxg_ctx_s* c;
   int cls;
+  hipStream_t st;
   hipEvent_t stop = nullptr;
-  ProfScope(mxg_ctx_s* ctx, int k);
+  ProfScope(mxg_ctx_s* ctx, int k, hipStream_t s = nullptr);
   ~ProfScope();
 };
 }  // namespace mxg

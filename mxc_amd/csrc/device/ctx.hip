@@ -23,6 +23,7 @@ hipEvent_t take_event(mxg_ctx_s* c) {
 
 void drain(mxg_ctx_s* c) {
   MXG_HIP(hipStreamSynchronize(c->stream));
+  MXG_HIP(hipStreamSynchronize(c->aux));
   for (auto& t : c->timed) {
     for (auto& pr : t.pending) {
       float ms = 0.f;
@@ -37,16 +38,16 @@ void drain(mxg_ctx_s* c) {
 }
 }  // namespace
 
-ProfScope::ProfScope(mxg_ctx_s* ctx, int k) : c(ctx), cls(k) {
+ProfScope::ProfScope(mxg_ctx_s* ctx, int k, hipStream_t s) : c(ctx), cls(k), st(s ? s : ctx->stream) {
   if (!c->profiling || cls < 0 || cls >= MXG_PROF_CLASSES) return;
   hipEvent_t start = take_event(c);
   stop = take_event(c);
-  MXG_HIP(hipEventRecord(start, c->stream));
+  MXG_HIP(hipEventRecord(start, st));
   c->timed[cls].pending.emplace_back(start, stop);
 }
 
 ProfScope::~ProfScope() {
-  if (stop) (void)hipEventRecord(stop, c->stream);
+  if (stop) (void)hipEventRecord(stop, st);
 }
 }  // namespace mxg
 
@@ -74,9 +75,16 @@ int mxg_init(int device, mxg_ctx_t* ctx) {
     auto* c = new mxg_ctx_s();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
     if (e != hipSuccess) {
+      if (c->join) (void)hipEventDestroy(c->join);
+      if (c->fork) (void)hipEventDestroy(c->fork);
+      if (c->aux) (void)hipStreamDestroy(c->aux);
+      if (c->stream) (void)hipStreamDestroy(c->stream);
       delete c;
-      throw Error(MXG_ERR_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+      throw Error(MXG_ERR_HIP, std::string("stream or event creation failed: ") + hipGetErrorString(e));
     }
     *ctx = c;
   });
@@ -117,6 +125,10 @@ int mxg_finalize(mxg_ctx_t ctx) {
     drain(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     MXG_HIP(hipStreamSynchronize(ctx->stream));
+    MXG_HIP(hipStreamSynchronize(ctx->aux));
+    MXG_HIP(hipEventDestroy(ctx->fork));
+    MXG_HIP(hipEventDestroy(ctx->join));
+    MXG_HIP(hipStreamDestroy(ctx->aux));
     MXG_HIP(hipStreamDestroy(ctx->stream));
     delete ctx;
   });

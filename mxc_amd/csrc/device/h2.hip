@@ -501,8 +501,8 @@ struct Ops<float> {
   static DevBuf<float>& t(mxg_h2_s* h) { return h->t32; }
 };
 
-// Device jobs of one precision: the upward pass per level, and one list with
-// the far terms of every level and the near terms of the leaf level. The
+// Device jobs of one precision: the upward pass per level, one list with the
+// far terms of every level and one with the near terms of the leaf level. The
 // vectors of a level hold MAX_RHS right-hand sides, lenZ (or n, or the term
 // count) apart.
 template <class T>
@@ -536,10 +536,12 @@ void ensure_matvec_jobs(mxg_h2_s* h) {
     }
     pair_list_upload(Lv.up_pl[pi], up, s);
   }
-  Level& Lf = h->lv[L];
-  pair_jobs_append<T, T>(prod, Lf.near_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, h->n, tb.p + Lf.tbaseN,
-                         Lf.lenTN, h->tlen);
   pair_list_upload(h->prod_pl[pi], prod, s);
+  Level& Lf = h->lv[L];
+  std::vector<PairJob> near;
+  pair_jobs_append<T, T>(near, Lf.near_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, h->n, tb.p + Lf.tbaseN,
+                         Lf.lenTN, h->tlen);
+  pair_list_upload(h->near_pl[pi], near, s);
   MXG_HIP(hipStreamSynchronize(s));
   h->jobs_ready[pi] = true;
 }
@@ -575,17 +577,28 @@ void h2_matvec_levels(mxg_h2_s* h, T* y, long long ldy, int nrhs) {
   ensure_matvec_jobs<T>(h);
   T* tb = Ops<T>::t(h).p;
   const long long lt = h->tlen;
+  // the leaf near terms depend on x alone: they run on the second stream,
+  // beside the upward pass and the far terms, which leave most of the GPU idle
+  hipStream_t sa = h->ctx->aux;
+  MXG_HIP(hipEventRecord(h->ctx->fork, s));
+  MXG_HIP(hipStreamWaitEvent(sa, h->ctx->fork, 0));
+  {
+    ProfScope prof(h->ctx, f64 ? MXG_PROF_MATVEC_LEAF_FP64 : MXG_PROF_MATVEC_LEAF_FP32, sa);
+    pair_gemv<T, T>(h->near_pl[pi], nrhs, sa);
+  }
+  MXG_HIP(hipEventRecord(h->ctx->join, sa));
   // upward pass: z_i = Q_i[:, :k]^T x_i
   for (long long l = L; l >= 0; l--) {
     if (h->lv[l].lenZ == 0) continue;
     ProfScope prof(h->ctx, MXG_PROF_MATVEC_UPPER);
     pair_gemv<T, T>(h->lv[l].up_pl[pi], nrhs, s);
   }
-  // every far term of every level and the near terms of the leaf, one launch
+  // every far term of every level, one launch
   {
-    ProfScope prof(h->ctx, f64 ? MXG_PROF_MATVEC_LEAF_FP64 : MXG_PROF_MATVEC_LEAF_FP32);
+    ProfScope prof(h->ctx, MXG_PROF_MATVEC_FAR);
     pair_gemv<T, T>(h->prod_pl[pi], nrhs, s);
   }
+  MXG_HIP(hipStreamWaitEvent(s, h->ctx->join, 0));
   // downward pass: w = w_in + far terms, y = Q w (+ near terms at the leaf)
   for (long long l = 0; l <= L; l++) {
     Level& Lv = h->lv[l];

@@ -87,11 +87,12 @@ struct mxg_h2_s {
   std::vector<mxg::DevBuf<float>> Z32, W32;
   mxg::DevBuf<float> x32, y32;
   bool shadow32 = false;
-  // matvec: input copy, product terms of every level, and the one launch that
-  // computes all far terms and the leaf near terms ([0] FP64, [1] FP32)
+  // matvec: input copy, product terms of every level, the launch of every
+  // level's far terms and the launch of the leaf near terms ([0] FP64, [1] FP32)
   mxg::DevBuf<double> x64, t64;
   mxg::DevBuf<float> t32;
-  mxg::PairList prod_pl[2];
+  mxg::PairList prod_pl[2];  // coupling products of every level
+  mxg::PairList near_pl[2];  // leaf near products
   bool jobs_ready[2] = {false, false};
   long long tlen = 1;  // term count of the matvec (the stride between right-hand sides)
 };
