@@ -513,14 +513,14 @@ void ensure_matvec_jobs(mxg_h2_s* h) {
   const long long L = h->L;
   long long tlen = 0;
   for (long long l = 0; l <= L; l++) {
-    h->lv[l].tbaseF = tlen;
-    tlen += h->lv[l].lenTF;
+    h->lv[l].tbaseF[pi] = tlen;
+    tlen += pi ? h->lv[l].lenTF32 : h->lv[l].lenTF;
   }
-  h->lv[L].tbaseN = tlen;
+  h->lv[L].tbaseN[pi] = tlen;
   tlen += h->lv[L].lenTN;
-  h->tlen = std::max<long long>(tlen, 1);
+  h->tlen[pi] = std::max<long long>(tlen, 1);
   DevBuf<T>& tb = Ops<T>::t(h);
-  tb.alloc((size_t)h->tlen * MAX_RHS);
+  tb.alloc((size_t)h->tlen[pi] * MAX_RHS);
   if constexpr (pi == 0) h->x64.reserve((size_t)h->n * MAX_RHS);
   std::vector<PairJob> prod;
   for (long long l = 0; l <= L; l++) {
@@ -531,16 +531,17 @@ void ensure_matvec_jobs(mxg_h2_s* h) {
     if (Lv.lenZ > 0) {
       pair_jobs_append<T, T>(up, Lv.up_t, Ops<T>::QT(Lv), (long long)Lv.offQT[Lv.nb], xin, Lv.lenX, lzin, Ops<T>::Z(h, l),
                              Lv.lenZ, Lv.lenZ);
-      pair_jobs_append<T, T>(prod, Lv.far_t, Ops<T>::C(Lv), (long long)Lv.C.n, Ops<T>::Z(h, l), Lv.lenZ, Lv.lenZ,
-                             tb.p + Lv.tbaseF, Lv.lenTF, h->tlen);
+      pair_jobs_append<T, T>(prod, pi ? Lv.far_t32 : Lv.far_t, Ops<T>::C(Lv), pi ? (long long)Lv.C32.n : (long long)Lv.C.n,
+                             Ops<T>::Z(h, l), Lv.lenZ, Lv.lenZ, tb.p + Lv.tbaseF[pi], pi ? Lv.lenTF32 : Lv.lenTF,
+                             h->tlen[pi]);
     }
     pair_list_upload(Lv.up_pl[pi], up, s);
   }
   pair_list_upload(h->prod_pl[pi], prod, s);
   Level& Lf = h->lv[L];
   std::vector<PairJob> near;
-  pair_jobs_append<T, T>(near, Lf.near_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, h->n, tb.p + Lf.tbaseN,
-                         Lf.lenTN, h->tlen);
+  pair_jobs_append<T, T>(near, Lf.near_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, h->n,
+                         tb.p + Lf.tbaseN[pi], Lf.lenTN, h->tlen[pi]);
   pair_list_upload(h->near_pl[pi], near, s);
   MXG_HIP(hipStreamSynchronize(s));
   h->jobs_ready[pi] = true;
@@ -576,7 +577,7 @@ void h2_matvec_levels(mxg_h2_s* h, T* y, long long ldy, int nrhs) {
   constexpr int pi = f64 ? 0 : 1;
   ensure_matvec_jobs<T>(h);
   T* tb = Ops<T>::t(h).p;
-  const long long lt = h->tlen;
+  const long long lt = h->tlen[pi];
   // the leaf near terms depend on x alone: they run on the second stream,
   // beside the upward pass and the far terms, which leave most of the GPU idle
   hipStream_t sa = h->ctx->aux;
@@ -598,17 +599,19 @@ void h2_matvec_levels(mxg_h2_s* h, T* y, long long ldy, int nrhs) {
     ProfScope prof(h->ctx, MXG_PROF_MATVEC_FAR);
     pair_gemv<T, T>(h->prod_pl[pi], nrhs, s);
   }
-  MXG_HIP(hipStreamWaitEvent(s, h->ctx->join, 0));
-  // downward pass: w = w_in + far terms, y = Q w (+ near terms at the leaf)
+  // downward pass: w = w_in + far terms, y = Q w (+ near terms at the leaf);
+  // only the leaf level waits for the near terms
   for (long long l = 0; l <= L; l++) {
     Level& Lv = h->lv[l];
     const bool leaf = (l == L);
+    if (leaf) MXG_HIP(hipStreamWaitEvent(s, h->ctx->join, 0));
     T* out = leaf ? y : Ops<T>::W(h, l + 1);
     const long long ldo = leaf ? ldy : h->lv[l + 1].lenZ;
     if (!leaf && ldo == 0) continue;
     ProfScope prof(h->ctx, MXG_PROF_MATVEC_UPPER);
-    comb_down<T>(Lv, Lv.vecQ[pi], Lv.d_toffF.p, Lv.d_toffN.p, Ops<T>::Q(Lv), Ops<T>::W(h, l), std::max<long long>(Lv.lenZ, 1),
-                 tb + Lv.tbaseF, leaf ? tb + Lv.tbaseN : nullptr, lt, out, ldo, nrhs, s);
+    comb_down<T>(Lv, Lv.vecQ[pi], pi ? Lv.d_toffF32.p : Lv.d_toffF.p, Lv.d_toffN.p, Ops<T>::Q(Lv), Ops<T>::W(h, l),
+                 std::max<long long>(Lv.lenZ, 1), tb + Lv.tbaseF[pi], leaf ? tb + Lv.tbaseN[pi] : nullptr, lt, out, ldo,
+                 nrhs, s);
   }
 }
 
@@ -645,6 +648,56 @@ using namespace mxg;
 
 namespace mxg {
 
+namespace {
+// One coupling block copied to FP32 with its rows padded to prow (zeros).
+struct PadTask {
+  long long src, dst;
+  int rows, prow, cols, pad;
+};
+
+__global__ void __launch_bounds__(256) k_pad_rows(const PadTask* __restrict__ tasks, const double* __restrict__ src,
+                                                  float* __restrict__ dst) {
+  const PadTask t = tasks[blockIdx.x];
+  const long long ne = (long long)t.prow * t.cols;
+  for (long long e = threadIdx.x; e < ne; e += blockDim.x) {
+    const int r = (int)(e % t.prow), c = (int)(e / t.prow);
+    dst[t.dst + e] = r < t.rows ? (float)src[t.src + r + (long long)c * t.rows] : 0.f;
+  }
+}
+}  // namespace
+
+// The FP32 coupling blocks with rows padded to a multiple of 4, their far
+// tasks and term offsets (each term holds the padded rows; the padding rows
+// of a product are zero and never read).
+void pad_couplings32(Level& Lv, hipStream_t s) {
+  const long long nb = Lv.nb, np = Lv.frp[nb];
+  Lv.offC32.assign(np + 1, 0);
+  Lv.toffF32.assign(np + 1, 0);
+  Lv.far_t32.clear();
+  std::vector<PadTask> pt;
+  for (long long i = 0; i < nb; i++) {
+    const long long k = Lv.ranks[i], kp = (k + 3) / 4 * 4;
+    for (long long p = Lv.frp[i]; p < Lv.frp[i + 1]; p++) {
+      const long long j = Lv.fcol[p];
+      Lv.offC32[p + 1] = Lv.offC32[p] + kp * Lv.ranks[j];
+      Lv.toffF32[p + 1] = Lv.toffF32[p] + kp;
+      if (k == 0) continue;
+      pt.push_back(PadTask{Lv.offC[p], Lv.offC32[p], (int)k, (int)kp, (int)Lv.ranks[j], 0});
+      Lv.far_t32.push_back(PairTask{Lv.offC32[p], Lv.offZ[j], Lv.toffF32[p], (int)kp, (int)Lv.ranks[j], (int)kp, 0});
+    }
+  }
+  Lv.lenTF32 = Lv.toffF32[np];
+  Lv.C32.alloc(std::max<long long>(Lv.offC32[np], 1));
+  Lv.d_toffF32.upload(Lv.toffF32, s);
+  if (pt.empty()) return;
+  MXG_REQUIRE(pt.size() < (1ull << 31), "too many coupling blocks");
+  DevBuf<PadTask> d;
+  d.upload(pt, s);
+  hipLaunchKernelGGL(k_pad_rows, dim3((unsigned)pt.size()), dim3(256), 0, s, d.p, Lv.C.p, Lv.C32.p);
+  MXG_HIP(hipGetLastError());
+  MXG_HIP(hipStreamSynchronize(s));
+}
+
 void make_shadow32(mxg_h2_s* h) {
   if (h->shadow32) return;
   hipStream_t s = h->ctx->stream;
@@ -654,8 +707,7 @@ void make_shadow32(mxg_h2_s* h) {
     Level& Lv = h->lv[l];
     Lv.Q32.alloc(Lv.Q.n);
     convert(Lv.Q32.p, Lv.Q.p, Lv.Q.n, s);
-    Lv.C32.alloc(Lv.C.n);
-    convert(Lv.C32.p, Lv.C.p, Lv.C.n, s);
+    pad_couplings32(Lv, s);
     Lv.QT32.alloc(Lv.QT.n);
     convert(Lv.QT32.p, Lv.QT.p, Lv.QT.n, s);
     if (l == h->L) {

@@ -59,18 +59,23 @@ struct Level {
   // blocks and its leading dimension, per near pair of this level (-1: none).
   std::vector<long long> up_off;
   std::vector<int> up_ld;
-  // FP32 copies of the operator data (inner matvec of mixed precision GMRES)
+  // FP32 copies of the operator data (inner matvec of mixed precision GMRES).
+  // C32 pads every coupling block's rows to a multiple of 4 with zeros
+  // (offsets offC32), so that its columns admit 16-byte loads.
   DevBuf<float> A32, Q32, C32;
+  std::vector<long long> offC32;
   // matvec tasks: up z_i = QT_i x_i (QT_i = Q_i[:, :k]^T, k x M), far
   // t_p = C_p z_j, leaf near u_p = A_p x_j; toffF / toffN: term offsets
-  // within the level's share of the term scratch (tbaseF / tbaseN)
-  std::vector<long long> offQT, toffF, toffN;
-  std::vector<PairTask> up_t, far_t, near_t;
+  // within the level's share of the term scratch (tbaseF / tbaseN; the FP32
+  // far terms have the padded row counts of C32: far_t32, toffF32)
+  std::vector<long long> offQT, toffF, toffN, toffF32;
+  std::vector<PairTask> up_t, far_t, near_t, far_t32;
   DevBuf<double> QT;
   DevBuf<float> QT32;
   PairList up_pl[2];  // [0] FP64, [1] FP32
-  DevBuf<long long> d_toffF, d_toffN;
-  long long lenTF = 0, lenTN = 0, tbaseF = 0, tbaseN = 0;
+  DevBuf<long long> d_toffF, d_toffN, d_toffF32;
+  long long lenTF = 0, lenTN = 0, lenTF32 = 0;
+  long long tbaseF[2] = {0, 0}, tbaseN[2] = {0, 0};  // [0] FP64, [1] FP32
   bool vecQ[2] = {false, false};  // every M_i admits 16-byte loads of Q columns (FP64 / FP32)
 };
 
@@ -94,7 +99,7 @@ struct mxg_h2_s {
   mxg::PairList prod_pl[2];  // coupling products of every level
   mxg::PairList near_pl[2];  // leaf near products
   bool jobs_ready[2] = {false, false};
-  long long tlen = 1;  // term count of the matvec (the stride between right-hand sides)
+  long long tlen[2] = {1, 1};  // term count of the matvec per precision (the stride between right-hand sides)
 };
 
 namespace mxg {
