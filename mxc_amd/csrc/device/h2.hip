@@ -529,8 +529,8 @@ void ensure_matvec_jobs(mxg_h2_s* h) {
     const T* xin = (l == L) ? Ops<T>::x(h) : Ops<T>::Z(h, l + 1);
     const long long lzin = (l == L) ? h->n : h->lv[l + 1].lenZ;
     if (Lv.lenZ > 0) {
-      pair_jobs_append<T, T>(up, Lv.up_t, Ops<T>::QT(Lv), (long long)Lv.offQT[Lv.nb], xin, Lv.lenX, lzin, Ops<T>::Z(h, l),
-                             Lv.lenZ, Lv.lenZ);
+      pair_jobs_append<T, T>(up, spread(Lv.up_t), Ops<T>::QT(Lv), (long long)Lv.offQT[Lv.nb], xin, Lv.lenX, lzin,
+                             Ops<T>::Z(h, l), Lv.lenZ, Lv.lenZ);
       pair_jobs_append<T, T>(prod, pi ? Lv.far_t32 : Lv.far_t, Ops<T>::C(Lv), pi ? (long long)Lv.C32.n : (long long)Lv.C.n,
                              Ops<T>::Z(h, l), Lv.lenZ, Lv.lenZ, tb.p + Lv.tbaseF[pi], pi ? Lv.lenTF32 : Lv.lenTF,
                              h->tlen[pi]);

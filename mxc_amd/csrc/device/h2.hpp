@@ -25,6 +25,24 @@ struct PairTask {
   int n, m, ld;
   int pad;
 };
+// The tasks with their rows cut into chunks of at most `rows` (row r0 of a
+// column-major block starts at b + r0): more, shorter jobs for a level whose
+// few blocks would otherwise leave the GPU idle. Every output row is still
+// written by exactly one job.
+inline std::vector<PairTask> split_rows(const std::vector<PairTask>& tasks, int rows) {
+  std::vector<PairTask> out;
+  for (const PairTask& t : tasks)
+    for (int r0 = 0; r0 < std::max(t.n, 1); r0 += rows)
+      out.push_back(PairTask{t.b + r0, t.v, t.out + r0, std::min(rows, t.n - r0), t.m, t.ld, t.pad});
+  return out;
+}
+// levels with fewer blocks than this get their rows split (chunks of 32)
+constexpr size_t SPLIT_BELOW = 1024;
+constexpr int SPLIT_ROWS = 32;
+inline std::vector<PairTask> spread(const std::vector<PairTask>& tasks) {
+  return tasks.size() < SPLIT_BELOW ? split_rows(tasks, SPLIT_ROWS) : tasks;
+}
+
 // The device form of a task: absolute addresses, and vec = 1 when the block
 // admits 16-byte column loads (address, rows and ld multiples of 16 bytes).
 struct PairJob {

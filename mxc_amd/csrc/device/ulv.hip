@@ -160,9 +160,9 @@ struct UlvImpl : UlvBase {
     const long long ylen = (l == h->L) ? h->n : std::max<long long>(h->lv[l + 1].lenZ, 1);
     const long long lx = std::max<long long>(H.lenX, 1), lz = std::max<long long>(H.lenZ, 1);
     std::vector<PairJob> jv, jf, jb;
-    pair_jobs_append<T, S>(jv, vt, Vstore(F), H.offQ[nb], y, ylen, ylen, F.X.p, lx, lx);
-    pair_jobs_append<T, S>(jf, ft, Astore(F), (long long)F.A.n, F.X.p, lx, lx, tbuf.p, tlen, tlen);
-    pair_jobs_append<T, S>(jb, bt, Astore(F), (long long)F.A.n, F.Z.p, lz, lz, tbuf.p, tlen, tlen);
+    pair_jobs_append<T, S>(jv, spread(vt), Vstore(F), H.offQ[nb], y, ylen, ylen, F.X.p, lx, lx);
+    pair_jobs_append<T, S>(jf, spread(ft), Astore(F), (long long)F.A.n, F.X.p, lx, lx, tbuf.p, tlen, tlen);
+    pair_jobs_append<T, S>(jb, spread(bt), Astore(F), (long long)F.A.n, F.Z.p, lz, lz, tbuf.p, tlen, tlen);
     pair_list_upload(F.v_pl, jv, s);
     pair_list_upload(F.fwd_pl, jf, s);
     pair_list_upload(F.bwd_pl, jb, s);
