@@ -2,7 +2,10 @@
 // solveDeviceGMRES, device_gmres.cu:26-115; algorithm of solver.cpp:144-201),
 // for a batch of right-hand sides solved in lockstep.
 //
-// Vectors (Krylov bases, residuals, iterates) are FP64 in HBM. Inner products
+// Vectors (Krylov bases, residuals, iterates) are FP64 in HBM, except the
+// Krylov basis of FP32 inner iterations (CGS2 mode), which is stored in FP32:
+// those iterations apply FP32 operators to it anyway, and the Arnoldi inner
+// products still accumulate in FP64 against the FP64 vector w. Inner products
 // use a two-stage reduction (per-block partial sums, then a fixed-order sum),
 // so results are bitwise reproducible run to run. Each cycle starts from the
 // true residual b - A x with the FP64 H^2 operator, which makes the restarts
@@ -36,12 +39,13 @@ constexpr int DOT_MAXC = 32;    // columns per pass of k_dots
 
 // Right-hand side q = blockIdx.y: part[(q * nblk + blk) * ncols + c] = sum
 // over the block's rows of V_q[:, c] * w_q (w read once per row).
-__global__ void __launch_bounds__(256) k_dots(const double* __restrict__ V, int ncols, long long ldv, long long ldq,
+template <class B>
+__global__ void __launch_bounds__(256) k_dots(const B* __restrict__ V, int ncols, long long ldv, long long ldq,
                                               long long n, const double* __restrict__ w, long long ldw,
                                               double* __restrict__ part) {
   __shared__ double red[4][DOT_MAXC];
   const int q = blockIdx.y;
-  const double* Vq = V + (long long)q * ldq;
+  const B* Vq = V + (long long)q * ldq;
   const double* wq = w + (long long)q * ldw;
   double* pq = part + ((long long)q * gridDim.x + blockIdx.x) * ncols;
   const long long r0 = (long long)blockIdx.x * DOT_ROWS;
@@ -56,7 +60,7 @@ __global__ void __launch_bounds__(256) k_dots(const double* __restrict__ V, int 
       const double wr = wq[r];
 #pragma unroll
       for (int c = 0; c < DOT_MAXC; c++)
-        if (c < nc) s[c] += Vq[r + (long long)(c0 + c) * ldv] * wr;
+        if (c < nc) s[c] += (double)Vq[r + (long long)(c0 + c) * ldv] * wr;
     }
 #pragma unroll
     for (int c = 0; c < DOT_MAXC; c++) {
@@ -87,39 +91,46 @@ __global__ void __launch_bounds__(256) k_sum_parts(const double* __restrict__ pa
   if (lane == 0) out[(long long)q * ldo + c] = s;
 }
 
-// y_q = x_q / sqrt(nrm2[q * ldn]) (zero when nrm2 is not positive)
+// y_q = x_q / sqrt(nrm2[q * ldn]) (zero when nrm2 is not positive), stored
+// as B; y2_q (when given) receives the stored value in FP64
+template <class B>
 __global__ void k_scale_rsqrt(const double* __restrict__ x, long long ldx, const double* __restrict__ nrm2, long long ldn,
-                              long long n, double* __restrict__ y, long long ldy) {
+                              long long n, B* __restrict__ y, long long ldy, double* __restrict__ y2, long long ldy2) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int q = blockIdx.y;
   const double v = nrm2[(long long)q * ldn];
-  if (r < n) y[r + q * ldy] = v > 0. ? x[r + q * ldx] / sqrt(v) : 0.;
+  if (r >= n) return;
+  const B b = (B)(v > 0. ? x[r + q * ldx] / sqrt(v) : 0.);
+  y[r + q * ldy] = b;
+  if (y2) y2[r + q * ldy2] = (double)b;
 }
 
 // w_q -= V_q[:, :ncols] h_q
-__global__ void k_sub_vh(const double* __restrict__ V, int ncols, long long ldv, long long ldq, long long n,
+template <class B>
+__global__ void k_sub_vh(const B* __restrict__ V, int ncols, long long ldv, long long ldq, long long n,
                          const double* __restrict__ h, long long ldh, double* __restrict__ w, long long ldw) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int q = blockIdx.y;
   if (r >= n) return;
-  const double* Vq = V + (long long)q * ldq;
+  const B* Vq = V + (long long)q * ldq;
   const double* hq = h + (long long)q * ldh;
   double s = w[r + q * ldw];
-  for (int c = 0; c < ncols; c++) s -= Vq[r + (long long)c * ldv] * hq[c];
+  for (int c = 0; c < ncols; c++) s -= (double)Vq[r + (long long)c * ldv] * hq[c];
   w[r + q * ldw] = s;
 }
 
 // x_q += V_q[:, :k_q] y_q
-__global__ void k_add_vy(const double* __restrict__ V, long long ldv, long long ldq, long long n,
+template <class B>
+__global__ void k_add_vy(const B* __restrict__ V, long long ldv, long long ldq, long long n,
                          const double* __restrict__ y, long long ldy, const int* __restrict__ k, double* __restrict__ x,
                          long long ldx) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int q = blockIdx.y;
   if (r >= n) return;
-  const double* Vq = V + (long long)q * ldq;
+  const B* Vq = V + (long long)q * ldq;
   const double* yq = y + (long long)q * ldy;
   double s = x[r + q * ldx];
-  for (int c = 0; c < k[q]; c++) s += Vq[r + (long long)c * ldv] * yq[c];
+  for (int c = 0; c < k[q]; c++) s += (double)Vq[r + (long long)c * ldv] * yq[c];
   x[r + q * ldx] = s;
 }
 
@@ -138,16 +149,23 @@ struct Gmres {
   hipStream_t s;
   int nblk;
   long long ldq, ldh;  // distance between right-hand sides: Krylov bases and per-step scalars
-  DevBuf<double> V, w, r, part, hd, yd;
+  bool f32;            // Krylov basis in FP32 (V32, with the newest column also in vcur)
+  DevBuf<double> V, w, r, part, hd, yd, vcur;
+  DevBuf<float> V32;
   DevBuf<int> kd;
   double* hpin = nullptr;  // pinned host copy of one Arnoldi step, ldh per right-hand side
 
-  Gmres(mxg_h2_s* a, mxg_ulv_t mm, int restart, int nrhs)
-      : A(a), M(mm), n(a->n), m(restart), nr(nrhs), s(a->ctx->stream) {
+  Gmres(mxg_h2_s* a, mxg_ulv_t mm, int restart, int nrhs, bool basis32)
+      : A(a), M(mm), n(a->n), m(restart), nr(nrhs), s(a->ctx->stream), f32(basis32) {
     nblk = (int)ceil_div(n, DOT_ROWS);
     ldq = n * (long long)(m + 1);
     ldh = 2 * (m + 1) + 1;
-    V.alloc((size_t)ldq * nr);
+    if (f32) {
+      V32.alloc((size_t)ldq * nr);
+      vcur.alloc((size_t)n * nr);
+    } else {
+      V.alloc((size_t)ldq * nr);
+    }
     w.alloc((size_t)n * nr);
     r.alloc((size_t)n * nr);
     part.alloc((size_t)nblk * (m + 1) * nr);
@@ -165,8 +183,9 @@ struct Gmres {
   dim3 grid() const { return dim3((unsigned)ceil_div(n, 256), (unsigned)nr); }
 
   // out_q[0..ncols) = V_q[:, :ncols]^T w_q on the device (out_q = out + q ldh)
-  void dots_dev(const double* Vp, int ncols, long long ldvq, const double* wp, long long ldw, double* out) {
-    hipLaunchKernelGGL(k_dots, dim3(nblk, nr), dim3(256), 0, s, Vp, ncols, n, ldvq, n, wp, ldw, part.p);
+  template <class B>
+  void dots_dev(const B* Vp, int ncols, long long ldvq, const double* wp, long long ldw, double* out) {
+    hipLaunchKernelGGL(k_dots<B>, dim3(nblk, nr), dim3(256), 0, s, Vp, ncols, n, ldvq, n, wp, ldw, part.p);
     MXG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sum_parts, dim3((ncols + 3) / 4, nr), dim3(256), 0, s, part.p, nblk, ncols, out, ldh);
     MXG_HIP(hipGetLastError());
@@ -177,6 +196,43 @@ struct Gmres {
     dots_dev(x, 1, ldx, x, ldx, hd.p);
     MXG_HIP(hipMemcpyAsync(hpin, hd.p, (size_t)ldh * nr * sizeof(double), hipMemcpyDeviceToHost, s));
     MXG_HIP(hipStreamSynchronize(s));
+  }
+
+  // the Krylov basis: out = V[:, :ncols]^T w; w -= V[:, :ncols] h; column col
+  // = x / sqrt(nrm2); x += V[:, :k] y (yd, kd); and column i as a matvec input
+  void basis_dots(int ncols, double* out) {
+    if (f32)
+      dots_dev<float>(V32.p, ncols, ldq, w.p, n, out);
+    else
+      dots_dev<double>(V.p, ncols, ldq, w.p, n, out);
+  }
+  void basis_sub(int ncols, const double* h) {
+    if (f32)
+      hipLaunchKernelGGL(k_sub_vh<float>, grid(), dim3(256), 0, s, V32.p, ncols, n, ldq, n, h, ldh, w.p, n);
+    else
+      hipLaunchKernelGGL(k_sub_vh<double>, grid(), dim3(256), 0, s, V.p, ncols, n, ldq, n, h, ldh, w.p, n);
+    MXG_HIP(hipGetLastError());
+  }
+  void basis_set(int col, const double* x, long long ldx, const double* nrm2) {
+    if (f32)
+      hipLaunchKernelGGL(k_scale_rsqrt<float>, grid(), dim3(256), 0, s, x, ldx, nrm2, ldh, n, V32.p + (size_t)col * n, ldq,
+                         vcur.p, n);
+    else
+      hipLaunchKernelGGL(k_scale_rsqrt<double>, grid(), dim3(256), 0, s, x, ldx, nrm2, ldh, n, V.p + (size_t)col * n, ldq,
+                         (double*)nullptr, (long long)0);
+    MXG_HIP(hipGetLastError());
+  }
+  void basis_add(double* x, long long ldx) {
+    if (f32)
+      hipLaunchKernelGGL(k_add_vy<float>, grid(), dim3(256), 0, s, V32.p, n, ldq, n, yd.p, (long long)(m + 1), kd.p, x, ldx);
+    else
+      hipLaunchKernelGGL(k_add_vy<double>, grid(), dim3(256), 0, s, V.p, n, ldq, n, yd.p, (long long)(m + 1), kd.p, x, ldx);
+    MXG_HIP(hipGetLastError());
+  }
+  // column i (the newest one) as an FP64 operator input, and its stride
+  const double* basis_col(int i, long long* ld) const {
+    *ld = f32 ? n : ldq;
+    return f32 ? vcur.p : V.p + (size_t)i * n;
   }
 
   void precondition(double* x) { ulv_apply(M, x, n, nr); }
@@ -237,7 +293,7 @@ struct Rhs {
 void gmres_batch(mxg_h2_s* A, mxg_ulv_t M, const mxg_gmres_opts_t* o, int nr, const double* d_b, long long ldb,
                  double* d_x, long long ldx, double* resid, long long* iters_out, long long* inner_out) {
   mxg_h2_s* Ain = o->inner_operator;
-  Gmres g(A, M, o->restart, nr);
+  Gmres g(A, M, o->restart, nr, o->mode == MXG_GMRES_CGS2 && o->inner_precision == MXG_FP32);
   const long long n = g.n;
   const int m = o->restart;
   const long long R = o->max_cycles + 1;
@@ -296,30 +352,26 @@ void gmres_batch(mxg_h2_s* A, mxg_ulv_t M, const mxg_gmres_opts_t* o, int nr, co
     for (int q = 0; q < nr; q++) any = any || st[q].active;
     if (!any) break;
     // V_q[:, 0] = r_q / beta_q (|r_q|^2 is still in hd)
-    hipLaunchKernelGGL(k_scale_rsqrt, g.grid(), dim3(256), 0, g.s, g.r.p, n, g.hd.p, g.ldh, n, g.V.p, g.ldq);
-    MXG_HIP(hipGetLastError());
+    g.basis_set(0, g.r.p, n, g.hd.p);
     for (int i = 0; i < m; i++) {
       bool open = false;
       for (int q = 0; q < nr; q++) open = open || st[q].open;
       if (!open) break;
-      double* vi = g.V.p + (size_t)i * n;
+      long long ldvi = 0;
+      const double* vi = g.basis_col(i, &ldvi);
       double* h1d = g.hd.p;
       double* h2d = g.hd.p + (m + 1);
       double* nrd = g.hd.p + 2 * (m + 1);
-      g.matvec(vi, g.ldq, g.w.p, o->inner_precision, Ain);
+      g.matvec(vi, ldvi, g.w.p, o->inner_precision, Ain);
       g.precondition(g.w.p);
-      g.dots_dev(g.V.p, i + 1, g.ldq, g.w.p, n, h1d);
-      hipLaunchKernelGGL(k_sub_vh, g.grid(), dim3(256), 0, g.s, g.V.p, i + 1, n, g.ldq, n, h1d, g.ldh, g.w.p, n);
-      MXG_HIP(hipGetLastError());
+      g.basis_dots(i + 1, h1d);
+      g.basis_sub(i + 1, h1d);
       if (!ref) {
-        g.dots_dev(g.V.p, i + 1, g.ldq, g.w.p, n, h2d);
-        hipLaunchKernelGGL(k_sub_vh, g.grid(), dim3(256), 0, g.s, g.V.p, i + 1, n, g.ldq, n, h2d, g.ldh, g.w.p, n);
-        MXG_HIP(hipGetLastError());
+        g.basis_dots(i + 1, h2d);
+        g.basis_sub(i + 1, h2d);
       }
-      g.dots_dev(g.w.p, 1, n, g.w.p, n, nrd);
-      hipLaunchKernelGGL(k_scale_rsqrt, g.grid(), dim3(256), 0, g.s, g.w.p, n, nrd, g.ldh, n, g.V.p + (size_t)(i + 1) * n,
-                         g.ldq);
-      MXG_HIP(hipGetLastError());
+      g.dots_dev<double>(g.w.p, 1, n, g.w.p, n, nrd);
+      g.basis_set(i + 1, g.w.p, n, nrd);
       // one transfer per step: h, h2 and |w|^2 of every right-hand side
       MXG_HIP(hipMemcpyAsync(g.hpin, g.hd.p, (size_t)g.ldh * nr * sizeof(double), hipMemcpyDeviceToHost, g.s));
       MXG_HIP(hipStreamSynchronize(g.s));
@@ -379,8 +431,7 @@ void gmres_batch(mxg_h2_s* A, mxg_ulv_t M, const mxg_gmres_opts_t* o, int nr, co
     }
     MXG_HIP(hipMemcpyAsync(g.yd.p, yh.data(), yh.size() * sizeof(double), hipMemcpyHostToDevice, g.s));
     MXG_HIP(hipMemcpyAsync(g.kd.p, kh.data(), kh.size() * sizeof(int), hipMemcpyHostToDevice, g.s));
-    hipLaunchKernelGGL(k_add_vy, g.grid(), dim3(256), 0, g.s, g.V.p, n, g.ldq, n, g.yd.p, (long long)(m + 1), g.kd.p, d_x, ldx);
-    MXG_HIP(hipGetLastError());
+    g.basis_add(d_x, ldx);
     // true residual of every right-hand side
     g.matvec(d_x, ldx, g.r.p, MXG_FP64);
     hipLaunchKernelGGL(k_b_minus, g.grid(), dim3(256), 0, g.s, d_b, ldb, n, g.r.p);
