@@ -61,7 +61,7 @@ def pct(a, q):
 
 
 def measured_traffic(nr):
-    """HBM bytes per launch of the FP32 product launch with nr right-hand
+    """HBM bytes per launch of the FP32 near-product launch with nr right-hand
     sides, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes (None when
     absent or measured for another batch width)."""
     path = os.path.join(ROOT, "profiles", "pair_gemv_matvec_fp32_traffic.md")

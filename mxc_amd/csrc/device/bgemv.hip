@@ -5,17 +5,18 @@
 // dense products t = B v over a list of blocks (a box's basis, its coupling
 // or near blocks, or sub-blocks of them), followed by a per-box sum of the
 // terms that land on the same box. The terms are computed pair-parallel: one
-// 256-thread workgroup per block, so a level's work spreads over as many
-// workgroups as it has blocks, whatever its number of boxes. Within a block,
-// lanes run down the rows (16-byte vector loads of a column when the block
-// is aligned) and the four waves take every fourth column; short columns
+// 256-thread workgroup per block above 64 KiB, one wave per smaller block
+// (four to a workgroup), so a level's work spreads over as many workgroups
+// as it has blocks, whatever its number of boxes. Within a block, lanes run
+// down the rows (16-byte vector loads of a column when the block is aligned,
+// 8-byte for bfloat16) and the waves take every fourth column; short columns
 // share a wave. Each loaded column is applied to NR right-hand sides, so a
 // block is streamed from HBM once per batch; the NR values of a column sit
 // side by side in LDS and reach every lane in one broadcast read. Each row
 // is a chain of fused multiply-adds down the wave's columns in order, the
-// four waves' partial rows are summed through LDS in a fixed order, and the
-// per-box sums read the terms back in CSR order: every result is
-// deterministic and does not depend on NR.
+// four waves' partial rows are summed through LDS in a fixed order (a lane
+// butterfly within a one-wave job), and the per-box sums read the terms back
+// in CSR order: every result is deterministic and does not depend on NR.
 //
 // Blocks are stored in the compute type T (double or float) or, for the
 // FP32 substitution of a preconditioner, in bfloat16 (the upper 16 bits of

@@ -7,14 +7,19 @@
 //          children's R, and (for a factorable matrix) each parent near block
 //          receives the couplings of its far child pairs (h2matrix.cpp:204-233);
 //   basis (compute_basis, h2matrix.cpp:13-63) for every box at once:
-//          G = K(far sample, skeleton) -> Householder R -> Eigen-rule CPQR ->
+//          G = K(far sample, skeleton) -> row reduction (a Rademacher sketch
+//          Omega G, or the Householder R of the reference) -> Eigen-rule CPQR ->
 //          rank, T = R11^{-1} R12, skeleton permutation ->
 //          orthogonal: QR of triu(Q_in) P [I; T^T] gives Q (M x M) and R;
 //          interpolative: Q = [P [I; T^T], 0], R = I;
 //   couplings (h2matrix.cpp:222-242): C = triu(R_y) K(S_y, S_x) triu(R_x)^T
 //          (orthogonal) or K(S_y, S_x) (interpolative).
-// The matvec (h2matrix.cpp:251-344) runs as one kernel per level and pass,
-// one workgroup per box.
+// The matvec (h2matrix.cpp:251-344) runs as lists of block products
+// (bgemv.hip): the upward pass level by level, the coupling products of
+// every level in one launch and, on the context's second stream beside
+// them, the leaf near products in one launch; then the downward per-box
+// sums level by level, of which only the leaf level waits for the near
+// products.
 #include "h2.hpp"
 
 #include <algorithm>

@@ -1,5 +1,7 @@
 // ULV factorization and substitution of an H^2 matrix on the device, in FP64
-// or FP32 (templated on the factor precision T).
+// or FP32 (templated on the factor precision T); an FP32 factorization may
+// keep bfloat16 copies of its factors (S) for the substitution, which still
+// accumulates in FP32.
 //
 // Factorization of one level (H2Matrix::factorize, h2matrix.cpp:346-425, and
 // its batched formulation device_factorize.cu:56-142), for boxes i with
@@ -21,7 +23,9 @@
 // z_i = x_i[:s] - sum_j A_ij.sr x_j[s_j:]; backward 0 -> L,
 // x_i = [w_i ; x_i[s:] - sum_j A_ij.rs w_j], y_i = U_i x_i. Vectors hand over
 // between levels in place (a level's z is its parent's y, the parent's y is
-// the child level's w), as in the matvec.
+// the child level's w), as in the matvec. Each level's products run as lists
+// of block products for up to 8 right-hand sides at once (bgemv.hip), the
+// sums of the terms per box in one more launch.
 #include <chrono>
 #include <memory>
 
