@@ -377,6 +377,23 @@ __global__ void __launch_bounds__(256) k_pair_gemv(const PairJob* __restrict__ j
     wg_gemv<T, S, 1, NR>(B, t.ld, t.n, t.m, vf, ef, red, vs);
 }
 
+// sum over p in [p0, p1) of t[toff[p] + off]: four independent partial sums,
+// so that four loads are in flight, added in a fixed order
+template <class T>
+__device__ inline T sum_terms(const T* __restrict__ t, const long long* __restrict__ toff, long long p0, long long p1,
+                              long long off) {
+  T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+  long long p = p0;
+  for (; p + 3 < p1; p += 4) {
+    a0 += t[toff[p] + off];
+    a1 += t[toff[p + 1] + off];
+    a2 += t[toff[p + 2] + off];
+    a3 += t[toff[p + 3] + off];
+  }
+  for (; p < p1; p++) a0 += t[toff[p] + off];
+  return (a0 + a1) + (a2 + a3);
+}
+
 // Strides (elements) between right-hand sides of the vectors of a per-box sum.
 struct Strides {
   long long a, b, c, d;
@@ -402,14 +419,14 @@ __global__ void __launch_bounds__(256) k_comb_down(const int* __restrict__ dims,
   for (int e = threadIdx.x; e < k * NR; e += 256) {
     const int r = e % k, q = e / k;
     T s = w_in[offZ[i] + r + q * ld.a];
-    for (long long p = frp[i]; p < frp[i + 1]; p++) s += tF[toffF[p] + r + q * ld.b];
+    s += sum_terms(tF, toffF, frp[i], frp[i + 1], r + q * ld.b);
     w[r + q * kmax] = s;
   }
   __syncthreads();
   T* yi = y + offX[i];
   const long long n0 = tN ? nrp[i] : 0, n1 = tN ? nrp[i + 1] : 0;
   auto emit = [&](int r, int q, T s) {
-    for (long long p = n0; p < n1; p++) s += tN[toffN[p] + r + q * ld.b];
+    if (n1 > n0) s += sum_terms(tN, toffN, n0, n1, r + q * ld.b);
     yi[r + q * ld.d] = s;
   };
   if (k == 0) {
@@ -432,7 +449,7 @@ __global__ void __launch_bounds__(256) k_comb_fwd(const int* __restrict__ ranks,
   for (int e = threadIdx.x; e < s * nr; e += 256) {
     const int r = e % s, q = e / s;
     T acc = x[offX[i] + r + q * ld.a];
-    for (long long p = nrp[i]; p < nrp[i + 1]; p++) acc -= t[toff[p] + r + q * ld.b];
+    acc -= sum_terms(t, toff, nrp[i], nrp[i + 1], r + q * ld.b);
     z[offZ[i] + r + q * ld.c] = acc;
   }
 }
@@ -445,13 +462,17 @@ __global__ void __launch_bounds__(256) k_comb_bwd(const int* __restrict__ dims, 
                                                   const long long* __restrict__ offZ, const long long* __restrict__ nrp,
                                                   const long long* __restrict__ toff, const S* __restrict__ U,
                                                   const T* __restrict__ x, const T* __restrict__ w,
-                                                  const T* __restrict__ t, T* __restrict__ y, Strides ld, int mmax) {
+                                                  const T* __restrict__ t, T* __restrict__ y, Strides ld, int mmax,
+                                                  int rows) {
   __shared__ __align__(16) T red[lds_elems<T>()];
   __shared__ __align__(16) T vs[lds_elems<T>()];
   extern __shared__ __align__(16) unsigned char dsm[];
   T* xv = reinterpret_cast<T*>(dsm);  // mmax x NR
   const int i = blockIdx.x;
+  const int r0 = blockIdx.y * rows;  // this workgroup's rows of y_i: r0 .. r0 + rows - 1
   const int M = dims[i], s = ranks[i];
+  if (r0 >= M) return;
+  const int Mc = min(rows, M - r0);
   for (int e = threadIdx.x; e < M * NR; e += 256) {
     const int r = e % M, q = e / M;
     T acc;
@@ -459,13 +480,13 @@ __global__ void __launch_bounds__(256) k_comb_bwd(const int* __restrict__ dims, 
       acc = w[offZ[i] + r + q * ld.b];
     } else {
       acc = x[offX[i] + r + q * ld.a];
-      for (long long p = nrp[i]; p < nrp[i + 1]; p++) acc -= t[toff[p] + (r - s) + q * ld.c];
+      acc -= sum_terms(t, toff, nrp[i], nrp[i + 1], (r - s) + q * ld.c);
     }
     xv[r + q * mmax] = acc;
   }
   __syncthreads();
-  T* yi = y + offX[i];
-  wg_gemv<T, S, V, NR>(U + offQ[i], M, M, M, [&](int c, int q) { return xv[c + q * mmax]; },
+  T* yi = y + offX[i] + r0;
+  wg_gemv<T, S, V, NR>(U + offQ[i] + r0, M, Mc, M, [&](int c, int q) { return xv[c + q * mmax]; },
                        [&](int r, int q, T v) { yi[r + q * ld.d] = v; }, red, vs);
 }
 
@@ -619,10 +640,12 @@ void comb_bwd(const Level& Lv, bool vec, const long long* d_toff, const S* U, co
     const T* wi = w + c0 * ldw;
     const T* ti = t + c0 * ldt;
     T* yo = y + c0 * ldy;
-    const dim3 g((unsigned)Lv.nb), b(256);
+    // levels with few boxes split each box's product into chunks of rows
+    const int rows = (size_t)Lv.nb < SPLIT_BELOW ? SPLIT_ROWS : mmax;
+    const dim3 g((unsigned)Lv.nb, (unsigned)((mmax + rows - 1) / rows)), b(256);
 #define MXG_COMB_BWD(VV, NN)                                                                                          \
   hipLaunchKernelGGL((k_comb_bwd<T, S, VV, NN>), g, b, shm, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p, Lv.d_offX.p,  \
-                     Lv.d_offZ.p, Lv.d_nrp.p, d_toff, U, xi, wi, ti, yo, ld, mmax)
+                     Lv.d_offZ.p, Lv.d_nrp.p, d_toff, U, xi, wi, ti, yo, ld, mmax, rows)
     if (vec) {
       switch (nr) {
         case 8: MXG_COMB_BWD(vwidth<S>(), 8); break;
