@@ -21,6 +21,29 @@ hipEvent_t take_event(mxg_ctx_s* c) {
   return e;
 }
 
+// CUs kept free of the second stream's work: the short dependent launches of
+// the context stream (the matvec's upward pass, one level at a time) start on
+// them at once instead of waiting for CUs that the leaf near products hold.
+// The runtime deals the mask bits out to the XCDs in turn (bit i to XCD i mod
+// 8 on a 256-CU MI355X), so clearing the lowest 32 bits frees 4 CUs on every
+// XCD; a mask that leaves some XCD no CU at all is not valid.
+constexpr int AUX_RESERVED_CUS = 32;
+
+// The second stream, masked to every CU but the first AUX_RESERVED_CUS, on a
+// device with more than 4 AUX_RESERVED_CUS CUs (the whole MI355X); a plain
+// stream on smaller partitions or where the runtime refuses the mask.
+hipError_t create_aux_stream(int device, hipStream_t* s) {
+  int ncu = 0;
+  hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+  if (e == hipSuccess && AUX_RESERVED_CUS > 0 && ncu > 4 * AUX_RESERVED_CUS) {
+    std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
+    for (int cu = AUX_RESERVED_CUS; cu < ncu; cu++) mask[cu / 32] |= 1u << (cu % 32);
+    if (hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data()) == hipSuccess) return hipSuccess;
+    (void)hipGetLastError();
+  }
+  return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
 void drain(mxg_ctx_s* c) {
   MXG_HIP(hipStreamSynchronize(c->stream));
   MXG_HIP(hipStreamSynchronize(c->aux));
@@ -75,7 +98,7 @@ int mxg_init(int device, mxg_ctx_t* ctx) {
     auto* c = new mxg_ctx_s();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = create_aux_stream(device, &c->aux);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming);
     if (e != hipSuccess) {
