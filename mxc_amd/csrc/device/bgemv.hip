@@ -501,7 +501,10 @@ __global__ void k_to_bf16(bf16_t* __restrict__ dst, const float* __restrict__ sr
 // the launch), on top of the 2 RED_BYTES of the block product: 64 KiB in all
 constexpr size_t DYN_LDS_BYTES = 65536 - 2 * RED_BYTES;
 
-// blocks up to this size (and with n <= 64 rows per vector lane) go to one wave
+// blocks up to this size (and with n <= 64 rows per vector lane) go to one
+// wave; bfloat16 blocks up to twice this size, so that a 256 x 256 leaf factor
+// of the substitution is one wave's job (each lane owns 4 whole rows, and no
+// reduction across waves is needed)
 constexpr long long WAVE_JOB_BYTES = 65536;
 
 // largest supported batch width not above nr
@@ -520,7 +523,8 @@ void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& t
     MXG_REQUIRE(t.n == 0 || t.m == 0 || t.b + (long long)(t.m - 1) * t.ld + t.n <= nB, "pair task outside its matrix");
     const S* b = B + t.b;
     const int vec = ((reinterpret_cast<uintptr_t>(b) % 16) == 0 && t.ld % W == 0 && t.n % W == 0) ? 1 : 0;
-    const int wave = (t.n <= 64 * (vec ? W : 1) && (long long)t.n * t.m * (long long)sizeof(S) <= WAVE_JOB_BYTES) ? 1 : 0;
+    const long long wave_bytes = std::is_same<S, bf16_t>::value ? 2 * WAVE_JOB_BYTES : WAVE_JOB_BYTES;
+    const int wave = (t.n <= 64 * (vec ? W : 1) && (long long)t.n * t.m * (long long)sizeof(S) <= wave_bytes) ? 1 : 0;
     jobs.push_back(PairJob{b, v + t.v, out + t.out, ldv, ldo, t.n, t.m, t.ld, vec, wave});
   }
 }
