@@ -454,6 +454,31 @@ __global__ void __launch_bounds__(256) k_comb_fwd(const int* __restrict__ ranks,
   }
 }
 
+// Backward substitution, the vector only: v_i = [w_i ; x_i[s:] - sum_near t_p]
+// into v (offsets offX), for a separate launch of the products U_i v_i; the
+// same arithmetic as the assembly in k_comb_bwd. Strides: a = x, b = w,
+// c = t, d = v.
+template <class T>
+__global__ void __launch_bounds__(256) k_bwd_gather(const int* __restrict__ dims, const int* __restrict__ ranks,
+                                                    const long long* __restrict__ offX, const long long* __restrict__ offZ,
+                                                    const long long* __restrict__ nrp, const long long* __restrict__ toff,
+                                                    const T* __restrict__ x, const T* __restrict__ w,
+                                                    const T* __restrict__ t, T* __restrict__ v, Strides ld, int nr) {
+  const int i = blockIdx.x;
+  const int M = dims[i], s = ranks[i];
+  for (int e = threadIdx.x; e < M * nr; e += 256) {
+    const int r = e % M, q = e / M;
+    T acc;
+    if (r < s) {
+      acc = w[offZ[i] + r + q * ld.b];
+    } else {
+      acc = x[offX[i] + r + q * ld.a];
+      acc -= sum_terms(t, toff, nrp[i], nrp[i + 1], (r - s) + q * ld.c);
+    }
+    v[offX[i] + r + q * ld.d] = acc;
+  }
+}
+
 // Backward substitution: v_i = [w_i ; x_i[s:] - sum_near t_p], y_i = U_i v_i.
 // Strides: a = x, b = w, c = t, d = y.
 template <class T, class S, int V, int NR>
@@ -628,6 +653,19 @@ template void comb_fwd<float>(const Level&, const long long*, const float*, long
                               long long, int, hipStream_t);
 template void comb_fwd<double>(const Level&, const long long*, const double*, long long, const double*, long long,
                                double*, long long, int, hipStream_t);
+
+template <class T>
+void bwd_gather(const Level& Lv, const long long* d_toff, const T* x, long long ldx, const T* w, long long ldw, const T* t,
+                long long ldt, T* v, long long ldv, int nrhs, hipStream_t s) {
+  if (Lv.nb == 0) return;
+  hipLaunchKernelGGL(k_bwd_gather<T>, dim3((unsigned)Lv.nb), dim3(256), 0, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offX.p,
+                     Lv.d_offZ.p, Lv.d_nrp.p, d_toff, x, w, t, v, Strides{ldx, ldw, ldt, ldv}, nrhs);
+  MXG_HIP(hipGetLastError());
+}
+template void bwd_gather<float>(const Level&, const long long*, const float*, long long, const float*, long long,
+                                const float*, long long, float*, long long, int, hipStream_t);
+template void bwd_gather<double>(const Level&, const long long*, const double*, long long, const double*, long long,
+                                 const double*, long long, double*, long long, int, hipStream_t);
 
 template <class T, class S>
 void comb_bwd(const Level& Lv, bool vec, const long long* d_toff, const S* U, const T* x, long long ldx, const T* w,

@@ -164,6 +164,10 @@ void comb_down(const Level& Lv, bool vec, const long long* d_toffF, const long l
 template <class T>
 void comb_fwd(const Level& Lv, const long long* d_toff, const T* x, long long ldx, const T* t, long long ldt, T* z,
               long long ldz, int nrhs, hipStream_t s);
+// v_i = [w_i ; x_i[s:] - sum_near t] (v at the offsets of x, stride ldv)
+template <class T>
+void bwd_gather(const Level& Lv, const long long* d_toff, const T* x, long long ldx, const T* w, long long ldw, const T* t,
+                long long ldt, T* v, long long ldv, int nrhs, hipStream_t s);
 // y_i = U_i [w_i ; x_i[s:] - sum_near t], U stored as S
 template <class T, class S>
 void comb_bwd(const Level& Lv, bool vec, const long long* d_toff, const S* U, const T* x, long long ldx, const T* w,

@@ -75,11 +75,14 @@ struct UlvImpl : UlvBase {
     // substitution task lists: x = V y per box; sr terms A_ij[:s_i, s_j:] x_j[s_j:]
     // (forward) and rs terms A_ij[s_i:, :s_j] w_j (backward) per near pair
     PairList v_pl, fwd_pl, bwd_pl;
+    // leaf only: y_i = U_i v_i over the gathered backward vectors (ubuf)
+    PairList u_pl;
     DevBuf<long long> d_toffU, d_toffB;
     bool vecM = false;
   };
   std::vector<Lv> lv;
   DevBuf<T> vin, tbuf;  // n and tlen entries per right-hand side
+  DevBuf<T> ubuf;       // the leaf's backward vectors [w_i ; x_i[s:] - sum t], n per right-hand side
   long long tlen = 1;
 
   explicit UlvImpl(mxg_h2_s* hh) : h(hh) {
@@ -110,6 +113,7 @@ struct UlvImpl : UlvBase {
       }
     }
     vin.alloc(h->n * MAX_RHS);
+    ubuf.alloc(h->n * MAX_RHS);
     long long tmax = 1;
     for (long long l = 0; l <= h->L; l++) {
       const Level& H = h->lv[l];
@@ -170,6 +174,13 @@ struct UlvImpl : UlvBase {
     pair_list_upload(F.v_pl, jv, s);
     pair_list_upload(F.fwd_pl, jf, s);
     pair_list_upload(F.bwd_pl, jb, s);
+    if (l == h->L) {
+      // the leaf's U products as their own launch: a 256 x 256 bfloat16 box
+      // factor is then one wave's job rather than one workgroup's
+      std::vector<PairJob> ju;
+      pair_jobs_append<T, S>(ju, spread(vt), Ustore(F), H.offQ[nb], ubuf.p, lx, lx, vin.p, h->n, h->n);
+      pair_list_upload(F.u_pl, ju, s);
+    }
     F.d_toffU.upload(toffU, s);
     F.d_toffB.upload(toffB, s);
     MXG_HIP(hipStreamSynchronize(s));
@@ -438,7 +449,12 @@ struct UlvImpl : UlvBase {
         const long long lo = (l == L) ? n : std::max<long long>(h->lv[l + 1].lenZ, 1);
         ProfScope prof(h->ctx, l == L ? MXG_PROF_ULV_BWD_LEAF : -1);
         pair_gemv<T, S>(F.bwd_pl, nr, s);
-        comb_bwd<T, S>(H, F.vecM, F.d_toffB.p, Ustore(F), F.X.p, lx, F.Z.p, lz, tbuf.p, tlen, out, lo, nr, s);
+        if (l == L) {
+          bwd_gather<T>(H, F.d_toffB.p, F.X.p, lx, F.Z.p, lz, tbuf.p, tlen, ubuf.p, lx, nr, s);
+          pair_gemv<T, S>(F.u_pl, nr, s);
+        } else {
+          comb_bwd<T, S>(H, F.vecM, F.d_toffB.p, Ustore(F), F.X.p, lx, F.Z.p, lz, tbuf.p, tlen, out, lo, nr, s);
+        }
       }
       convert2d(x + c0 * ldx, ldx, v, n, n, nr, s);
     }
