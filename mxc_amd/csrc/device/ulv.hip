@@ -25,7 +25,9 @@
 // between levels in place (a level's z is its parent's y, the parent's y is
 // the child level's w), as in the matvec. Each level's products run as lists
 // of block products for up to 8 right-hand sides at once (bgemv.hip), the
-// sums of the terms per box in one more launch.
+// sums of the terms per box in one more launch. At the leaf, the backward
+// sums write x_i to a buffer and the products U_i x_i run as their own list,
+// where a 256 x 256 bfloat16 factor is one wave's job.
 #include <chrono>
 #include <memory>
 
