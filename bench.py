@@ -115,7 +115,7 @@ def main():
     bs = args.batch
     rhs = [ctx.array(np.stack([uniform_vector(n, -1.0, 1.0, 1000 + k * bs + q) for q in range(bs)], axis=1))
            for k in range(args.warmup + args.steps)]
-    xs = [ctx.zeros((n, bs)) for _ in range(args.steps)]
+    sol = {path: [ctx.zeros((n, bs)) for _ in range(args.steps)] for path in ("fp64", "mixed")}
     work = ctx.zeros((n, bs))
 
     def solve(B, X, inner_prec):
@@ -124,7 +124,7 @@ def main():
         res, it, inner = A.gmres_batch(M, B, X, tol, restart, max_cycles, N.GMRES_CGS2, inner_prec, inner_op)
         return [float(r[-1]) for r in res], [int(v) for v in it], [int(v) for v in inner]
 
-    def run_path(factor_prec, inner_prec, prof_class):
+    def run_path(factor_prec, inner_prec, xs):
         info = M.factorize(factor_prec)
         assert not info.any(), f"singular ULV blocks per level: {info}"
         fsec = M.factor_seconds()
@@ -171,8 +171,13 @@ def main():
         return dict(elapsed=elapsed, lat=lat, cycles=cycles, inner=inners, h2_resid=res, dense=dense, factor_s=fsec,
                     kernels=kern)
 
-    fp64 = run_path(N.MXG_FP64, N.MXG_FP64, 0)
-    mixed = run_path(N.MXG_BF16 if args.mixed_factor == "bf16" else N.MXG_FP32, N.MXG_FP32, 1)
+    fp64 = run_path(N.MXG_FP64, N.MXG_FP64, sol["fp64"])
+    mixed = run_path(N.MXG_BF16 if args.mixed_factor == "bf16" else N.MXG_FP32, N.MXG_FP32, sol["mixed"])
+    # error of every mixed-path solution against the FP64 path's for the same right-hand side
+    mix_err = []
+    for k in range(args.steps):
+        xm, x64 = sol["mixed"][k].download(), sol["fp64"][k].download()
+        mix_err += [float(np.linalg.norm(xm[:, q] - x64[:, q]) / np.linalg.norm(x64[:, q])) for q in range(bs)]
 
     # roofline of the dominant kernel of the mixed path: the FP32 launch of the
     # leaf near products of the matvec for the batch's right-hand sides.
@@ -227,6 +232,7 @@ def main():
         "mixed_path": {
             "cycles": mixed["cycles"], "inner_iters": mixed["inner"], "h2_relres_max": max(mixed["h2_resid"]),
             "dense_relres_max": max(mixed["dense"]), "factor_s": round(mixed["factor_s"], 4),
+            "rel_diff_vs_fp64_max": max(mix_err), "rel_diff_vs_fp64_median": float(np.median(mix_err)),
         },
         "setup_s": {"operator": round(t_op, 2), "preconditioner": round(t_pc, 2), "inner_operator": round(t_in, 2)},
         "kernels_mixed": mixed["kernels"],
@@ -242,6 +248,9 @@ def main():
             "traffic": measured_traffic(nr),
         },
     }
+    # release every device object before the CPU baseline and before exit, so
+    # that no HIP handle is left to static destructors at interpreter shutdown
+    ctx.close()
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.sampling, args.op_eps, n)

@@ -120,6 +120,28 @@ int mxg_h2_construct_level(mxg_h2_t h, long long level, long long nboxes, const 
 int mxg_h2_download(mxg_h2_t h, long long level, int what, double* dst, long long count);
 long long mxg_h2_level_size(mxg_h2_t h, long long level, int what);
 
+/* Level structure of an H^2 matrix built on the host, for callers that hand
+ * over their own level data instead of constructing on the device: the
+ * reference's createMatrixDesc + copyDataInMatrixDesc
+ * (device_precondition.cu:39-154,198-202) as used by
+ * H2MatrixSolver::factorizeDeviceM (solver.cpp:102-119), and the
+ * createSpMatrixDesc of allocSparseMV (device_csr_matrix.cu:164-225,
+ * solver.cpp:59-64). Levels are defined leaves first, as in construction:
+ * dims as in mxg_h2_construct_level, ranks[nboxes] the skeleton ranks
+ * (0 at the root), near/far the level-local CSR lists. The leaf's skeleton
+ * indices are its bodies in tree order and its Q is the identity; every other
+ * datum starts at zero and is given with mxg_h2_upload. */
+int mxg_h2_define_level(mxg_h2_t h, long long level, long long nboxes, const long long* dims, const long long* ranks,
+                        const long long* near_rp, const long long* near_col, const long long* far_rp,
+                        const long long* far_col);
+/* Host data into a level that is constructed or defined (copyDataInMatrixDesc,
+ * device_precondition.cu:198-202). what and layout as in mxg_h2_download; an
+ * upper level's near blocks A hold the couplings of the far child pairs
+ * (h2matrix.cpp:172-173), the blocks of near child pairs are filled by the
+ * factorization. count = mxg_h2_level_size. Factorizations made before the
+ * upload keep their own copies and are not affected. */
+int mxg_h2_upload(mxg_h2_t h, long long level, int what, const double* src, long long count);
+
 /* H^2 matvec y = A x on the device (matVecDeviceH2, device_csr_matrix.cu:328-338).
  * x, y: tree-ordered FP64 vectors of length nbodies, nrhs columns with leading
  * dimension ld. precision MXG_FP32 applies an FP32 copy of the operator data

@@ -23,6 +23,9 @@ MXG_BF16 = 2  # ULV only: FP32 factorization, bfloat16-stored factors for the su
 KERNELS = {"laplace": 0, "yukawa": 1, "gaussian": 2}
 GMRES_REFERENCE = 0
 GMRES_CGS2 = 1
+MXG_H2_FACTORABLE = 1
+# level data kinds of mxg_h2_download / mxg_h2_upload
+H2_NEAR, H2_BASIS, H2_R, H2_COUPLING, H2_SKELETON = 0, 1, 2, 3, 4
 
 
 class MxgKernel(C.Structure):
@@ -88,6 +91,12 @@ def gpu() -> C.CDLL:
             "mxg_gen_block": [P, MxgKernel, c_ll, c_ll, P, P, P, c_ll],
             "mxg_dense_matvec": [P, MxgKernel, P, c_ll, P, P],
             "mxg_dense_matvec_batch": [P, MxgKernel, P, c_ll, P, c_ll, P, c_ll, c_ll],
+            "mxg_h2_create": [P, MxgKernel, P, c_ll, c_ll, c_int, C.POINTER(P)],
+            "mxg_h2_destroy": [P],
+            "mxg_h2_define_level": [P, c_ll, c_ll, P, P, P, P, P, P],
+            "mxg_h2_upload": [P, c_ll, c_int, P, c_ll],
+            "mxg_ulv_create": [P, c_int, C.POINTER(P)],
+            "mxg_ulv_destroy": [P],
             "mxg_h2_matvec": [P, P, P, c_ll, c_ll, c_int],
             "mxg_h2_download": [P, c_ll, c_int, P, c_ll],
             "mxg_ulv_solve": [P, P, c_ll, c_ll],

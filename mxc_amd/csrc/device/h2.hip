@@ -431,7 +431,8 @@ void upload_indices(mxg_h2_s* h, Level& Lv) {
 
 // Matvec tasks of a constructed level and the term offsets of its downward
 // pass (the device jobs are made per precision on first use).
-void build_matvec_lists(mxg_h2_s* h, Level& Lv, bool leaf) {
+// QT_i = Q_i[:, :k]^T of every box (the upward pass streams its columns).
+void basis_transpose(mxg_h2_s* h, Level& Lv) {
   hipStream_t s = h->ctx->stream;
   const long long nb = Lv.nb;
   Lv.offQT.assign(nb + 1, 0);
@@ -442,6 +443,13 @@ void build_matvec_lists(mxg_h2_s* h, Level& Lv, bool leaf) {
   hipLaunchKernelGGL(k_basis_transpose, dim3((unsigned)nb), dim3(256), 0, s, Lv.d_dims.p, Lv.d_ranks.p, Lv.d_offQ.p, d_offQT.p,
                      Lv.Q.p, Lv.QT.p);
   MXG_HIP(hipGetLastError());
+  MXG_HIP(hipStreamSynchronize(s));
+}
+
+void build_matvec_lists(mxg_h2_s* h, Level& Lv, bool leaf) {
+  hipStream_t s = h->ctx->stream;
+  const long long nb = Lv.nb;
+  basis_transpose(h, Lv);
   Lv.up_t.clear();
   Lv.far_t.clear();
   Lv.near_t.clear();
@@ -729,6 +737,135 @@ void make_shadow32(mxg_h2_s* h) {
   h->shadow32 = true;
 }
 
+namespace {
+// Box sizes, lists, offsets and zeroed arrays of level l (levels are set up
+// from the leaves upward: an upper level's box sizes must be the sums of its
+// children's ranks, and the leaves must cover every body).
+void level_structure(mxg_h2_s* h, long long l, long long nb, const long long* dims, const long long* near_rp,
+                     const long long* near_col, const long long* far_rp, const long long* far_col) {
+  MXG_REQUIRE(nb == (1ll << l), "nboxes must be 2^level");
+  MXG_REQUIRE(l == h->L || h->lv[l + 1].built, "set up levels from the leaves upward");
+  hipStream_t s = h->ctx->stream;
+  Level& Lv = h->lv[l];
+  Level* low = (l < h->L) ? &h->lv[l + 1] : nullptr;
+  Lv = Level();
+  Lv.nb = nb;
+  Lv.dims.assign(dims, dims + nb);
+  Lv.nrp.assign(near_rp, near_rp + nb + 1);
+  Lv.frp.assign(far_rp, far_rp + nb + 1);
+  MXG_REQUIRE(Lv.nrp[0] == 0 && Lv.frp[0] == 0, "CSR row pointers must start at 0");
+  for (long long i = 0; i < nb; i++)
+    MXG_REQUIRE(Lv.nrp[i + 1] >= Lv.nrp[i] && Lv.frp[i + 1] >= Lv.frp[i], "CSR row pointers must not decrease");
+  MXG_REQUIRE(Lv.nrp[nb] == 0 || near_col, "null near columns");
+  MXG_REQUIRE(Lv.frp[nb] == 0 || far_col, "null far columns");
+  Lv.ncol.assign(near_col, near_col + Lv.nrp[nb]);
+  Lv.fcol.assign(far_col, far_col + Lv.frp[nb]);
+  for (long long t = 0; t < Lv.nrp[nb]; t++) MXG_REQUIRE(Lv.ncol[t] >= 0 && Lv.ncol[t] < nb, "near column out of range");
+  for (long long t = 0; t < Lv.frp[nb]; t++) MXG_REQUIRE(Lv.fcol[t] >= 0 && Lv.fcol[t] < nb, "far column out of range");
+  for (long long i = 0; i < nb; i++) {
+    MXG_REQUIRE(dims[i] >= 0 && dims[i] <= 4096, "box dimension out of range (max 4096)");
+    if (low) MXG_REQUIRE(dims[i] == low->ranks[2 * i] + low->ranks[2 * i + 1], "upper dims must be the sum of the child ranks");
+  }
+  if (!low) {
+    long long total = 0;
+    for (long long i = 0; i < nb; i++) total += dims[i];
+    MXG_REQUIRE(total == h->n, "leaf dims must sum to nbodies");
+  }
+  Lv.ranks.assign(nb, 0);
+  Lv.offX.assign(nb + 1, 0);
+  Lv.offQ.assign(nb + 1, 0);
+  Lv.diag.assign(nb, -1);
+  for (long long i = 0; i < nb; i++) {
+    Lv.offX[i + 1] = Lv.offX[i] + dims[i];
+    Lv.offQ[i + 1] = Lv.offQ[i] + dims[i] * dims[i];
+    Lv.maxM = std::max(Lv.maxM, dims[i]);
+    Lv.diag[i] = lookup(Lv.nrp, Lv.ncol, i, i);
+    MXG_REQUIRE(Lv.diag[i] >= 0, "every box needs its diagonal near block");
+  }
+  Lv.lenX = Lv.offX[nb];
+  const long long nnzA = Lv.nrp[nb];
+  Lv.offA.assign(nnzA + 1, 0);
+  for (long long i = 0; i < nb; i++)
+    for (long long t = Lv.nrp[i]; t < Lv.nrp[i + 1]; t++) Lv.offA[t + 1] = Lv.offA[t] + dims[i] * dims[Lv.ncol[t]];
+  Lv.up_off.assign(nnzA, -1);
+  Lv.up_ld.assign(nnzA, 0);
+  Lv.S.alloc(std::max<long long>(Lv.lenX, 1));
+  Lv.Q.alloc_zero(Lv.offQ[nb], s);
+  Lv.R.alloc_zero(Lv.offQ[nb], s);
+  if (!low || (h->flags & MXG_H2_FACTORABLE)) Lv.A.alloc_zero(Lv.offA[nnzA], s);
+}
+
+// For every near pair (i, j) of level l and child pair (y, x) of it: a near
+// child pair hands its skeleton block to the parent block at up_off
+// (factorizeCopyNext, h2matrix.cpp:427-446; LowerIndA of comm-mpi.cpp:117-149);
+// with copy_far, a far child pair's coupling block is copied into the parent
+// near block (h2matrix.cpp:172-173), when the parent keeps its near blocks.
+void link_children(mxg_h2_s* h, Level& Lv, Level& low, bool copy_far) {
+  std::vector<const double*> cs;
+  std::vector<double*> cd;
+  std::vector<int> cm, cn, cls, cld;
+  const bool keepA = Lv.A.n > 0;
+  for (long long i = 0; i < Lv.nb; i++) {
+    const long long M = Lv.dims[i];
+    for (int cy = 0; cy < 2; cy++) {
+      const long long y = 2 * i + cy;
+      const long long offy = cy ? low.ranks[2 * i] : 0, ny = low.ranks[y];
+      if (ny == 0) continue;
+      for (long long ij = Lv.nrp[i]; ij < Lv.nrp[i + 1]; ij++) {
+        const long long j = Lv.ncol[ij];
+        for (int cx = 0; cx < 2; cx++) {
+          const long long x = 2 * j + cx;
+          const long long offx = cx ? low.ranks[2 * j] : 0, nx = low.ranks[x];
+          if (nx == 0) continue;
+          const long long lowN = lookup(low.nrp, low.ncol, y, x);
+          const long long lowC = lookup(low.frp, low.fcol, y, x);
+          const long long dp = Lv.offA[ij] + offy + offx * M;
+          if (lowN >= 0) {
+            low.up_off[lowN] = dp;
+            low.up_ld[lowN] = (int)M;
+          } else if (lowC >= 0 && keepA && copy_far) {
+            cs.push_back(low.C.p + low.offC[lowC]);
+            cd.push_back(Lv.A.p + dp);
+            cm.push_back((int)ny); cn.push_back((int)nx); cls.push_back((int)ny); cld.push_back((int)M);
+          }
+        }
+      }
+    }
+  }
+  copy2d_batched<double>(cs, cd, cm, cn, cls, cld, h->ctx->stream);
+}
+
+// Layout that depends on the ranks: skeleton vectors and zeroed couplings.
+void level_ranks(mxg_h2_s* h, Level& Lv) {
+  const long long nb = Lv.nb;
+  Lv.offZ.assign(nb + 1, 0);
+  for (long long i = 0; i < nb; i++) {
+    Lv.offZ[i + 1] = Lv.offZ[i] + Lv.ranks[i];
+    Lv.maxK = std::max(Lv.maxK, Lv.ranks[i]);
+  }
+  Lv.lenZ = Lv.offZ[nb];
+  const long long nnzC = Lv.frp[nb];
+  Lv.offC.assign(nnzC + 1, 0);
+  for (long long i = 0; i < nb; i++)
+    for (long long t = Lv.frp[i]; t < Lv.frp[i + 1]; t++) Lv.offC[t + 1] = Lv.offC[t] + Lv.ranks[i] * Lv.ranks[Lv.fcol[t]];
+  Lv.C.alloc_zero(Lv.offC[nnzC], h->ctx->stream);
+}
+
+// Index arrays, matvec lists and vector buffers of a level whose data is set.
+void level_finish(mxg_h2_s* h, Level& Lv, bool leaf) {
+  hipStream_t s = h->ctx->stream;
+  const long long l = (long long)(&Lv - h->lv.data());
+  upload_indices(h, Lv);
+  build_matvec_lists(h, Lv, leaf);
+  h->jobs_ready[0] = h->jobs_ready[1] = false;
+  h->Z[l].alloc_zero(std::max<long long>(Lv.lenZ, 1) * MAX_RHS, s);
+  h->W[l].alloc_zero(std::max<long long>(Lv.lenZ, 1) * MAX_RHS, s);
+  MXG_HIP(hipStreamSynchronize(s));
+  Lv.built = true;
+  h->shadow32 = false;
+}
+}  // namespace
+
 }  // namespace mxg
 
 extern "C" {
@@ -776,61 +913,15 @@ int mxg_h2_construct_level(mxg_h2_t h, long long l, long long nb, const long lon
   return guarded([&] {
     MXG_REQUIRE(h && dims && near_rp && far_rp && ranks_out, "null argument");
     MXG_REQUIRE(l >= 0 && l <= h->L, "level out of range");
-    MXG_REQUIRE(nb == (1ll << l), "nboxes must be 2^level");
-    MXG_REQUIRE(l == h->L || h->lv[l + 1].built, "construct levels from the leaves upward");
     MXG_REQUIRE(l < h->L || leaf_begin, "leaf level needs leaf_begin");
     MXG_HIP(hipSetDevice(h->ctx->device));
     hipStream_t s = h->ctx->stream;
+    level_structure(h, l, nb, dims, near_rp, near_col, far_rp, far_col);
     Level& Lv = h->lv[l];
     Level* low = (l < h->L) ? &h->lv[l + 1] : nullptr;
-    Lv = Level();
-    Lv.nb = nb;
-    Lv.dims.assign(dims, dims + nb);
-    Lv.nrp.assign(near_rp, near_rp + nb + 1);
-    Lv.ncol.assign(near_col, near_col + Lv.nrp[nb]);
-    Lv.frp.assign(far_rp, far_rp + nb + 1);
-    Lv.fcol.assign(far_col, far_col + Lv.frp[nb]);
-    MXG_REQUIRE(Lv.nrp[0] == 0 && Lv.frp[0] == 0, "CSR row pointers must start at 0");
-    for (long long t = 0; t < Lv.nrp[nb]; t++) MXG_REQUIRE(Lv.ncol[t] >= 0 && Lv.ncol[t] < nb, "near column out of range");
-    for (long long t = 0; t < Lv.frp[nb]; t++) MXG_REQUIRE(Lv.fcol[t] >= 0 && Lv.fcol[t] < nb, "far column out of range");
-    for (long long i = 0; i < nb; i++) {
-      MXG_REQUIRE(dims[i] >= 0 && dims[i] <= 4096, "box dimension out of range (max 4096)");
-      if (low) MXG_REQUIRE(dims[i] == low->ranks[2 * i] + low->ranks[2 * i + 1], "upper dims must be the sum of the child ranks");
-    }
     if (!low) {
-      long long total = 0;
-      for (long long i = 0; i < nb; i++) {
+      for (long long i = 0; i < nb; i++)
         MXG_REQUIRE(leaf_begin[i] >= 0 && leaf_begin[i] + dims[i] <= h->n, "leaf body range out of bounds");
-        total += dims[i];
-      }
-      MXG_REQUIRE(total == h->n, "leaf dims must sum to nbodies");
-    }
-    Lv.ranks.assign(nb, 0);
-    Lv.offX.assign(nb + 1, 0);
-    Lv.offQ.assign(nb + 1, 0);
-    Lv.diag.assign(nb, -1);
-    for (long long i = 0; i < nb; i++) {
-      Lv.offX[i + 1] = Lv.offX[i] + dims[i];
-      Lv.offQ[i + 1] = Lv.offQ[i] + dims[i] * dims[i];
-      Lv.maxM = std::max(Lv.maxM, dims[i]);
-      Lv.diag[i] = lookup(Lv.nrp, Lv.ncol, i, i);
-      MXG_REQUIRE(Lv.diag[i] >= 0, "every box needs its diagonal near block");
-    }
-    Lv.lenX = Lv.offX[nb];
-    const long long nnzA = Lv.nrp[nb];
-    Lv.offA.assign(nnzA + 1, 0);
-    for (long long i = 0; i < nb; i++)
-      for (long long t = Lv.nrp[i]; t < Lv.nrp[i + 1]; t++) Lv.offA[t + 1] = Lv.offA[t] + dims[i] * dims[Lv.ncol[t]];
-    Lv.up_off.assign(nnzA, -1);
-    Lv.up_ld.assign(nnzA, 0);
-
-    Lv.S.alloc(std::max<long long>(Lv.lenX, 1));
-    Lv.Q.alloc_zero(Lv.offQ[nb], s);
-    Lv.R.alloc_zero(Lv.offQ[nb], s);
-    const bool keepA = !low || (h->flags & MXG_H2_FACTORABLE);
-    if (keepA) Lv.A.alloc_zero(Lv.offA[nnzA], s);
-
-    if (!low) {
       DevBuf<long long> d_begin;
       d_begin.upload(leaf_begin, nb, s);
       upload_indices(h, Lv);
@@ -846,14 +937,12 @@ int mxg_h2_construct_level(mxg_h2_t h, long long l, long long nb, const long lon
         }
       gen_tasks<double>(gens, h->bodies.p, h->k, s);
     } else {
+      // skeleton = the children's skeleton prefixes, Q_in = blockdiag of the children's R
       std::vector<const int*> ss;
       std::vector<int*> sd;
       std::vector<const double*> qs;
       std::vector<double*> qd;
       std::vector<int> sm, sn, sls, sld, qm, qn, qls, qld;
-      std::vector<const double*> cs;
-      std::vector<double*> cd;
-      std::vector<int> cm, cn, cls, cld;
       for (long long i = 0; i < nb; i++) {
         const long long M = dims[i];
         for (int cy = 0; cy < 2; cy++) {
@@ -866,55 +955,48 @@ int mxg_h2_construct_level(mxg_h2_t h, long long l, long long nb, const long lon
           qs.push_back(low->R.p + low->offQ[y]);
           qd.push_back(Lv.Q.p + Lv.offQ[i] + offy + offy * M);
           qm.push_back((int)ny); qn.push_back((int)ny); qls.push_back((int)low->dims[y]); qld.push_back((int)M);
-          for (long long ij = Lv.nrp[i]; ij < Lv.nrp[i + 1]; ij++) {
-            const long long j = Lv.ncol[ij];
-            for (int cx = 0; cx < 2; cx++) {
-              const long long x = 2 * j + cx;
-              const long long offx = cx ? low->ranks[2 * j] : 0, nx = low->ranks[x];
-              if (nx == 0) continue;
-              const long long lowN = lookup(low->nrp, low->ncol, y, x);
-              const long long lowC = lookup(low->frp, low->fcol, y, x);
-              const long long dp = Lv.offA[ij] + offy + offx * M;
-              if (lowN >= 0) {
-                low->up_off[lowN] = dp;
-                low->up_ld[lowN] = (int)M;
-              } else if (lowC >= 0 && keepA) {
-                cs.push_back(low->C.p + low->offC[lowC]);
-                cd.push_back(Lv.A.p + dp);
-                cm.push_back((int)ny); cn.push_back((int)nx); cls.push_back((int)ny); cld.push_back((int)M);
-              }
-            }
-          }
         }
       }
       copy2d_batched<int>(ss, sd, sm, sn, sls, sld, s);
       copy2d_batched<double>(qs, qd, qm, qn, qls, qld, s);
-      copy2d_batched<double>(cs, cd, cm, cn, cls, cld, s);
+      link_children(h, Lv, *low, true);
     }
-
     construct_basis(h, Lv, l, far_off, far_idx, tol, maxrank, orth);
-
-    Lv.offZ.assign(nb + 1, 0);
-    for (long long i = 0; i < nb; i++) {
-      Lv.offZ[i + 1] = Lv.offZ[i] + Lv.ranks[i];
-      Lv.maxK = std::max(Lv.maxK, Lv.ranks[i]);
-      ranks_out[i] = Lv.ranks[i];
-    }
-    Lv.lenZ = Lv.offZ[nb];
-    const long long nnzC = Lv.frp[nb];
-    Lv.offC.assign(nnzC + 1, 0);
-    for (long long i = 0; i < nb; i++)
-      for (long long t = Lv.frp[i]; t < Lv.frp[i + 1]; t++) Lv.offC[t + 1] = Lv.offC[t] + Lv.ranks[i] * Lv.ranks[Lv.fcol[t]];
-    Lv.C.alloc_zero(Lv.offC[nnzC], s);
+    for (long long i = 0; i < nb; i++) ranks_out[i] = Lv.ranks[i];
+    level_ranks(h, Lv);
     construct_coupling(h, Lv, orth);
-    upload_indices(h, Lv);
-    build_matvec_lists(h, Lv, !low);
-    h->jobs_ready[0] = h->jobs_ready[1] = false;
-    h->Z[l].alloc_zero(std::max<long long>(Lv.lenZ, 1) * MAX_RHS, s);
-    h->W[l].alloc_zero(std::max<long long>(Lv.lenZ, 1) * MAX_RHS, s);
-    MXG_HIP(hipStreamSynchronize(s));
-    Lv.built = true;
-    h->shadow32 = false;
+    level_finish(h, Lv, !low);
+  });
+}
+
+int mxg_h2_define_level(mxg_h2_t h, long long l, long long nb, const long long* dims, const long long* ranks,
+                        const long long* near_rp, const long long* near_col, const long long* far_rp,
+                        const long long* far_col) {
+  return guarded([&] {
+    MXG_REQUIRE(h && dims && ranks && near_rp && far_rp, "null argument");
+    MXG_REQUIRE(l >= 0 && l <= h->L, "level out of range");
+    MXG_HIP(hipSetDevice(h->ctx->device));
+    hipStream_t s = h->ctx->stream;
+    for (long long i = 0; i < nb; i++) MXG_REQUIRE(ranks[i] >= 0 && ranks[i] <= dims[i], "rank out of range (0..dim)");
+    if (l == 0) for (long long i = 0; i < nb; i++) MXG_REQUIRE(ranks[i] == 0, "the root has rank 0");
+    level_structure(h, l, nb, dims, near_rp, near_col, far_rp, far_col);
+    Level& Lv = h->lv[l];
+    Level* low = (l < h->L) ? &h->lv[l + 1] : nullptr;
+    if (!low) {
+      // the leaf's skeleton is its bodies in tree order, Q_in = I (h2matrix.cpp:179-188)
+      upload_indices(h, Lv);
+      DevBuf<long long> d_begin;
+      d_begin.upload(Lv.offX.data(), nb, s);
+      hipLaunchKernelGGL(k_leaf_init, dim3((unsigned)nb), dim3(256), 0, s, (int)nb, Lv.d_offX.p, Lv.d_offQ.p, Lv.d_dims.p,
+                         d_begin.p, Lv.S.p, Lv.Q.p);
+      MXG_HIP(hipGetLastError());
+      MXG_HIP(hipStreamSynchronize(s));
+    } else {
+      link_children(h, Lv, *low, false);
+    }
+    Lv.ranks.assign(ranks, ranks + nb);
+    level_ranks(h, Lv);
+    level_finish(h, Lv, !low);
   });
 }
 
@@ -950,6 +1032,34 @@ int mxg_h2_download(mxg_h2_t h, long long l, int what, double* dst, long long co
     const double* src = what == 0 ? Lv.A.p : what == 1 ? Lv.Q.p : what == 2 ? Lv.R.p : Lv.C.p;
     MXG_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, s));
     MXG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int mxg_h2_upload(mxg_h2_t h, long long l, int what, const double* src, long long count) {
+  return guarded([&] {
+    MXG_REQUIRE(h && (src || count == 0), "null argument");
+    MXG_REQUIRE(l >= 0 && l <= h->L && h->lv[l].built, "level not set up");
+    const long long n = mxg_h2_level_size(h, l, what);
+    MXG_REQUIRE(n >= 0 && count == n, "count does not match the level size");
+    MXG_HIP(hipSetDevice(h->ctx->device));
+    if (n == 0) return;
+    Level& Lv = h->lv[l];
+    hipStream_t s = h->ctx->stream;
+    if (what == 4) {
+      std::vector<int> tmp(n);
+      for (long long i = 0; i < n; i++) {
+        MXG_REQUIRE(src[i] >= 0 && src[i] < (double)h->n && src[i] == std::floor(src[i]), "body index out of range");
+        tmp[i] = (int)src[i];
+      }
+      MXG_HIP(hipMemcpyAsync(Lv.S.p, tmp.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+      MXG_HIP(hipStreamSynchronize(s));
+      return;
+    }
+    double* dst = what == 0 ? Lv.A.p : what == 1 ? Lv.Q.p : what == 2 ? Lv.R.p : Lv.C.p;
+    MXG_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s));
+    MXG_HIP(hipStreamSynchronize(s));
+    if (what == 1) basis_transpose(h, Lv);
+    h->shadow32 = false;  // FP32 copies are made again on their next use
   });
 }
 

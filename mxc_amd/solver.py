@@ -210,3 +210,81 @@ class H2:
             self.close()
         except Exception:
             pass
+
+
+class HostLevels:
+    """An H^2 matrix handed over level by level from host data, as the
+    reference's H2MatrixSolver does when it factorizes on the device
+    (factorizeDeviceM, solver.cpp:102-119: createMatrixDesc +
+    copyDataInMatrixDesc, device_precondition.cu:39-154,198-202).
+
+    levels[l] (l = 0 .. L) is a dict with the level-local structure "dims",
+    "ranks", "near" = (rowptr, col), "far" = (rowptr, col), and the data the
+    caller has: "A" (near blocks; at an upper level the couplings of the far
+    child pairs), "Q" (full M x M bases), "R", "C" (couplings), each in the
+    layout of mxg_h2_download. Vectors are tree-ordered over problem.bodies."""
+
+    def __init__(self, problem: Problem, levels, kernel="laplace", diag=1e3, alpha=1.0, factorable=True):
+        self.problem = problem
+        self.levels = len(levels) - 1
+        assert self.levels == problem.levels
+        ctx = problem.ctx
+        h = C.c_void_p()
+        N.check_gpu(N.gpu().mxg_h2_create(ctx.handle, kernel_desc(kernel, diag, alpha), problem.bodies.ctypes.data,
+                                          problem.n, self.levels, N.MXG_H2_FACTORABLE if factorable else 0, C.byref(h)))
+        self.handle = h.value
+        self.ulv = None
+        ctx.adopt(self, Context.SOLVERS)
+        i64 = lambda a: np.ascontiguousarray(a, dtype=np.int64)  # noqa: E731
+        for l in range(self.levels, -1, -1):
+            lv = levels[l]
+            dims, ranks = i64(lv["dims"]), i64(lv["ranks"])
+            (nrp, ncol), (frp, fcol) = (tuple(map(i64, lv["near"])), tuple(map(i64, lv["far"])))
+            N.check_gpu(N.gpu().mxg_h2_define_level(self.handle, l, len(dims), dims.ctypes.data, ranks.ctypes.data,
+                                                    nrp.ctypes.data, ncol.ctypes.data, frp.ctypes.data, fcol.ctypes.data))
+            for key, what in (("A", N.H2_NEAR), ("Q", N.H2_BASIS), ("R", N.H2_R), ("C", N.H2_COUPLING)):
+                if key in lv:
+                    self.upload(l, what, lv[key])
+
+    def upload(self, level: int, what: int, data) -> None:
+        a = np.ascontiguousarray(data, dtype=np.float64)
+        N.check_gpu(N.gpu().mxg_h2_upload(self.handle, level, what, a.ctypes.data, a.size))
+
+    def download(self, level: int, what: int) -> np.ndarray:
+        n = N.gpu().mxg_h2_level_size(self.handle, level, what)
+        out = np.zeros(max(n, 0))
+        N.check_gpu(N.gpu().mxg_h2_download(self.handle, level, what, out.ctypes.data, n))
+        return out
+
+    def matvec(self, x: DeviceArray, y: DeviceArray, precision: int = N.MXG_FP64) -> None:
+        nrhs = x.shape[1] if len(x.shape) > 1 else 1
+        N.check_gpu(N.gpu().mxg_h2_matvec(self.handle, x.ptr, y.ptr, nrhs, self.problem.n, precision))
+
+    def factorize(self, precision: int = N.MXG_FP64) -> np.ndarray:
+        if self.ulv:
+            N.gpu().mxg_ulv_destroy(self.ulv)
+            self.ulv = None
+        f = C.c_void_p()
+        N.check_gpu(N.gpu().mxg_ulv_create(self.handle, precision, C.byref(f)))
+        self.ulv = f.value
+        info = np.zeros(self.levels + 1, np.int32)
+        N.check_gpu(N.gpu().mxg_ulv_info(self.ulv, info.ctypes.data))
+        return info
+
+    def precondition(self, x: DeviceArray) -> None:
+        nrhs = x.shape[1] if len(x.shape) > 1 else 1
+        N.check_gpu(N.gpu().mxg_ulv_solve(self.ulv, x.ptr, nrhs, self.problem.n))
+
+    def close(self):
+        if self.handle and self.problem.ctx.handle:
+            if self.ulv:
+                N.gpu().mxg_ulv_destroy(self.ulv)
+            N.gpu().mxg_h2_destroy(self.handle)
+        self.handle = None
+        self.ulv = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

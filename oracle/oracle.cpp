@@ -922,12 +922,27 @@ void orc_h2_dims(const orc_h2* h, long long level, long long* dims, long long* r
     if (ranks) ranks[i] = Lv.ranks[i];
   }
 }
-// what: 1 = bases Q, 2 = R (M*M per box, column major, box order); returns the count
+// Level data in the layout of matrix_container.cpp:6-36 at one rank; returns
+// the count. what: 0 = near blocks A (M_i x M_j column major, in the order of
+// the level's near list; in place, so after orc_h2_factorize they hold the
+// factored blocks), 1 = bases Q, 2 = R (M*M per box; V after the
+// factorization), 3 = couplings C (k_i x k_j, in the order of the far list).
 long long orc_h2_download(const orc_h2* h, long long level, int what, double* out) {
   const Level& Lv = h->lv[level];
-  const std::vector<double>& v = (what == 1) ? Lv.Q : Lv.R;
+  const std::vector<double>& v = what == 0 ? Lv.A : what == 1 ? Lv.Q : what == 2 ? Lv.R : Lv.C;
   if (out) std::copy(v.begin(), v.end(), out);
   return (long long)v.size();
+}
+// Level-local near (far = 0) or far (far = 1) list (comm-mpi.cpp:105-115 at
+// P = 1): rowptr nb + 1 entries, columns box indices within the level. Fills
+// rp/col when given; returns the number of entries.
+long long orc_h2_level_list(const orc_h2* h, long long level, int far, long long* rp, long long* col) {
+  const Level& Lv = h->lv[level];
+  const std::vector<long>& r = far ? Lv.crp : Lv.arp;
+  const std::vector<long>& c = far ? Lv.ccol : Lv.acol;
+  if (rp) std::copy(r.begin(), r.end(), rp);
+  if (col) std::copy(c.begin(), c.end(), col);
+  return (long long)c.size();
 }
 long long orc_h2_near_count(const orc_h2* h, long long level) { return h->lv[level].arp.back(); }
 long long orc_h2_far_count(const orc_h2* h, long long level) { return h->lv[level].crp.back(); }

@@ -68,7 +68,10 @@ orc_h2* orc_h2_create(const orc_problem* p, int kid, double diag, double alpha, 
 void orc_h2_destroy(orc_h2* h);
 long long orc_h2_nboxes(const orc_h2* h, long long level);
 void orc_h2_dims(const orc_h2* h, long long level, long long* dims, long long* ranks);
+/* what: 0 = near blocks A, 1 = bases Q, 2 = R (V after orc_h2_factorize), 3 = couplings C */
 long long orc_h2_download(const orc_h2* h, long long level, int what, double* out);
+/* level-local near (far = 0) or far (far = 1) list; returns the entry count */
+long long orc_h2_level_list(const orc_h2* h, long long level, int far, long long* rp, long long* col);
 long long orc_h2_near_count(const orc_h2* h, long long level);
 long long orc_h2_far_count(const orc_h2* h, long long level);
 void orc_h2_matvec(orc_h2* h, double* x);

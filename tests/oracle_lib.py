@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
             "orc_h2_dims": (None, [P, c_ll, P, P]),
             "orc_h2_near_count": (c_ll, [P, c_ll]),
             "orc_h2_download": (c_ll, [P, c_ll, c_int, P]),
+            "orc_h2_level_list": (c_ll, [P, c_ll, c_int, P, P]),
             "orc_h2_far_count": (c_ll, [P, c_ll]),
             "orc_h2_matvec": (None, [P, P]),
             "orc_h2_factorize": (c_int, [P]),
@@ -161,6 +162,15 @@ class H2:
         out = np.zeros(lib().orc_h2_download(self.h, level, what, None))
         lib().orc_h2_download(self.h, level, what, ptr(out))
         return out
+
+    def level_list(self, level: int, far: bool):
+        """Level-local near (far=False) or far (far=True) CSR list: (rowptr, col)."""
+        nb = lib().orc_h2_nboxes(self.h, level)
+        nnz = lib().orc_h2_level_list(self.h, level, int(far), None, None)
+        rp = np.zeros(nb + 1, np.int64)
+        col = np.zeros(max(nnz, 1), np.int64)
+        lib().orc_h2_level_list(self.h, level, int(far), ptr(rp), ptr(col))
+        return rp, col[:nnz]
 
     def matvec(self, x: np.ndarray) -> np.ndarray:
         y = np.ascontiguousarray(x, dtype=np.float64).copy()

@@ -184,3 +184,45 @@ def test_gmres_reaches_dense_solution(small):
     assert np.all(np.diff(res[1:]) <= 0)
     xs = np.linalg.solve(K, b)
     assert O.rel_err(x, xs) < 1e-10
+
+
+@pytest.mark.parametrize("n,leaf,kernel,tol,rank", [
+    (256, 128, "laplace", 0.0, 24),
+    (257, 128, "yukawa", 1e-4, 60),
+    (200, 100, "laplace", 1e-6, 30),
+])
+def test_single_schur_ulv_of_two_near_boxes_is_exact(n, leaf, kernel, tol, rank):
+    """Pins the off-diagonal Schur updates of the factorization
+    (h2matrix.cpp:405-424): with two leaves that are near each other (L = 1),
+    every term of the single Schur complement onto the skeleton runs
+    (A_ii.sr A_ij.rs, A_ij.sr B_j, A_ij.sr rr_j^-1 A_ij.sr^T) and the root is a
+    full LU. The preconditioner is then exactly U A'^-1 U^T, where A' = U^T A U
+    (U = blockdiag of the boxes' full orthogonal bases, skeleton columns first)
+    with the r x r coupling between the two boxes' redundant parts removed:
+    the one interaction a single Schur complement leaves out."""
+    prob = O.Problem(O.cube_points(n, 11), leaf)
+    assert prob.levels == 1
+    M = prob.h2(kernel=kernel, alpha=1.0, tol=tol, rank=rank, orth=True, hss_far=True, sampling=0)
+    assert O.lib().orc_h2_near_count(M.h, 1) == 4  # the two leaves are near each other
+    d, s = M.dims(1)
+    assert np.all(s > 0) and np.all(s < d)
+    q = M.download(1, 1)
+    U = np.zeros((n, n))
+    o = oq = 0
+    for m in map(int, d):
+        U[o:o + m, o:o + m] = q[oq:oq + m * m].reshape(m, m, order="F")
+        o += m
+        oq += m * m
+    assert np.abs(U.T @ U - np.eye(n)).max() < 1e-12
+    K = kernel_dense(kernel, prob.bodies)
+    Ah = U.T @ K @ U
+    r0 = np.arange(s[0], d[0])
+    r1 = np.arange(d[0] + s[1], n)
+    Ah[np.ix_(r0, r1)] = 0.
+    Ah[np.ix_(r1, r0)] = 0.
+    b = O.uniform_vec(n, -1, 1, 5)
+    expect = U @ np.linalg.solve(Ah, U.T @ b)
+    assert M.factorize() == 0
+    assert O.rel_err(M.precondition(b), expect) < 1e-12
+    # the dropped coupling matters: the exact inverse of A differs
+    assert O.rel_err(np.linalg.solve(K, b), expect) > 1e-9
