@@ -53,6 +53,8 @@ def parse():
                     help="ULV of the mixed path: FP32 factorization with bfloat16-stored or FP32-stored factors")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=16384, help="points of the CPU baseline sample")
+    ap.add_argument("--cpu-phases", action="store_true",
+                    help="only time the oracle's phases on the host cores at N=20000 and N=65536 (no GPU work)")
     return ap.parse_args()
 
 
@@ -81,6 +83,9 @@ def log(msg):
 
 def main():
     args = parse()
+    if args.cpu_phases:
+        cpu_phases([(20000, 0), (65536, 1)])
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build targets one GPU (the H^2 solve path keeps the whole tree on one device); "
@@ -257,41 +262,122 @@ def main():
     print(json.dumps(out))
 
 
-def cpu_baseline(ns, sampling, op_eps, n_target):
-    """The CPU oracle (FP64 restatement of the reference) on the same workload
-    at ns points: construction and factorization untimed, then FP64 GMRES
-    solves (restart 30, tol 1e-10) timed on the host cores; solves/s scaled to
-    N = n_target by the O(N) cost of a solve."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
+def host_info():
+    """Host cores as the process sees them: nproc, OMP_NUM_THREADS, CPU model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"nproc": len(os.sched_getaffinity(0)), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "cpu_model": model}
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    threads = max(1, min(threads, 16))
-    pts = O.sphere_points(ns, 2)
-    P = O.Problem(pts, 256)
-    A = P.h2(kernel="yukawa", alpha=1.0, tol=op_eps, sampling=sampling, nthreads=threads)
-    M = P.h2(kernel="yukawa", alpha=1.0, tol=1e-6, rank=64, orth=True, hss_far=True, sampling=sampling, nthreads=threads)
-    M.factorize()
+
+def host_threads():
+    """Threads of the CPU runs: OMP_NUM_THREADS when set (the host's share of
+    cores), else every core the process may use."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, int(env)) if env and env.isdigit() else len(os.sched_getaffinity(0))
+
+
+def timed_solves(A, M, O, ns, seconds, max_solves):
     times = []
     t_all = time.perf_counter()
     k = 0
     while True:
         b = O.uniform_vec(ns, -1, 1, 1000 + k)
         ts = time.perf_counter()
-        x, res = A.gmres(M, b, 1e-10, 30, 20)
+        A.gmres(M, b, 1e-10, 30, 20)
         times.append(time.perf_counter() - ts)
         k += 1
-        if time.perf_counter() - t_all > 12.0 or k >= 200:
+        if time.perf_counter() - t_all > seconds or k >= max_solves:
             break
-    per = float(np.mean(times))
+    return float(np.mean(times)), k
+
+
+def cpu_baseline(ns, sampling, op_eps, n_target):
+    """The CPU oracle (FP64 restatement of the reference) on the metric's
+    workload at ns points (construction and factorization untimed): GMRES
+    solves (reference mode, restart 30, tol 1e-10) timed for about 10 s on
+    every host thread and for about 5 s on one thread. `value` is the
+    all-threads rate scaled to N = n_target by the O(N) cost of a solve; the
+    measured rates at ns are reported beside it."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    threads = host_threads()
+    pts = O.sphere_points(ns, 2)
+    P = O.Problem(pts, 256)
+    A = P.h2(kernel="yukawa", alpha=1.0, tol=op_eps, sampling=sampling, nthreads=threads)
+    M = P.h2(kernel="yukawa", alpha=1.0, tol=1e-6, rank=64, orth=True, hss_far=True, sampling=sampling, nthreads=threads)
+    M.factorize()
+    per, k = timed_solves(A, M, O, ns, 10.0, 200)
+    A.set_threads(1)
+    M.set_threads(1)
+    per1, k1 = timed_solves(A, M, O, ns, 5.0, 50)
     return {
         "value": round((1.0 / per) * ns / n_target, 6),
-        "unit": "solves/s (scaled to N=2^20)",
+        "unit": "solves/s (scaled to N=2^20 from the measured rate at the sample size)",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle GMRES solves (reference mode, CGS1, restart 30) at N={ns} sphere Yukawa, "
-                  f"{k} solves, {per * 1e3:.1f} ms/solve measured, scaled by {ns}/{n_target}",
+        "sample": f"oracle GMRES solves (reference mode, CGS1, restart 30, tol 1e-10) at N={ns}, unit-sphere Yukawa, "
+                  f"operator eps {op_eps:g}, preconditioner eps 1e-6 rank 64: {k} solves on {threads} threads "
+                  f"({per * 1e3:.1f} ms each), {k1} on 1 thread ({per1 * 1e3:.1f} ms each)",
+        "measured_solves_per_s_at_sample": round(1.0 / per, 4),
+        "measured_solves_per_s_at_sample_1thread": round(1.0 / per1, 4),
+        "host": host_info(),
     }
+
+
+def cpu_phases(sizes):
+    """The reference CPU path's phases timed with the oracle on the host cores
+    (lorasp N 2 256 1e-10 100 2000: cube points, Laplace with K(0) = 1e3,
+    b = A x_true): operator and preconditioner construction, factorization,
+    one substitution, GMRES (restart 10, 50 cycles) to 1e-10 and the dense
+    check, on every host thread; factorization, substitution and GMRES again
+    on one thread. Prints one JSON line per size."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    threads = host_threads()
+    for n, seed in sizes:
+        ph = {}
+        t = time.perf_counter()
+        pts = O.cube_points(n, seed)
+        P = O.Problem(pts, 256)
+        x = O.uniform_vec(n, 0, 1, 5489)
+        b = O.dense_matvec("laplace", 1e3, 0.0, P.bodies, x, threads)
+        ph["dense_matvec_s"] = time.perf_counter() - t
+        t = time.perf_counter()
+        A = P.h2(tol=1e-10, sampling=2000, nthreads=threads)
+        ph["operator_construct_s"] = time.perf_counter() - t
+        t = time.perf_counter()
+        M = P.h2(rank=100, orth=True, hss_far=True, sampling=2000, nthreads=threads)
+        ph["preconditioner_construct_s"] = time.perf_counter() - t
+        cerr = O.rel_err(A.matvec(x), b)
+        res1 = {}
+        for nt in (threads, 1):
+            M1 = M if nt == threads else P.h2(rank=100, orth=True, hss_far=True, sampling=2000, nthreads=threads)
+            M1.set_threads(nt)
+            A.set_threads(nt)
+            t = time.perf_counter()
+            assert M1.factorize() == 0
+            f = time.perf_counter() - t
+            t = time.perf_counter()
+            z = M1.precondition(b)
+            sub = time.perf_counter() - t
+            t = time.perf_counter()
+            xs, res = A.gmres(M1, b, 1e-10, 10, 50)
+            g = time.perf_counter() - t
+            res1[nt] = dict(factor_s=f, substitution_s=sub, gmres_s=g, gmres_cycles=len(res) - 1,
+                            gmres_residual=float(res[-1]), substitution_err=O.rel_err(z, x))
+        t = time.perf_counter()
+        rr = O.rel_err(O.dense_matvec("laplace", 1e3, 0.0, P.bodies, xs, threads), b)
+        ph["dense_check_s"] = time.perf_counter() - t
+        print(json.dumps({"N": n, "seed": seed, "threads": threads, "phases_all_threads": {**ph, **res1[threads]},
+                          "phases_1_thread": res1[1], "construct_err": cerr, "dense_relres": rr,
+                          "solution_err": O.rel_err(xs, x), "host": host_info()}), flush=True)
 
 
 if __name__ == "__main__":

@@ -156,7 +156,9 @@ int mxg_ulv_destroy(mxg_ulv_t f);
 int mxg_ulv_info(mxg_ulv_t f, int* info_per_level);
 /* x <- M^{-1} x in place (matSolvePreconditionDeviceH2, device_factorize.cu:244-254). */
 int mxg_ulv_solve(mxg_ulv_t f, double* d_x, long long nrhs, long long ld);
-/* seconds spent in the last factorization, measured with HIP events */
+/* seconds of the factorization (from the first to the last of its launches
+ * on the context stream, HIP events; the allocation and list building of
+ * mxg_ulv_create before it are not counted) */
 double mxg_ulv_factor_seconds(mxg_ulv_t f);
 
 /* ---- GMRES: replaces solveDeviceGMRES (device_gmres.cu:26-115) ---- */

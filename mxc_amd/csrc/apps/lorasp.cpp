@@ -8,7 +8,8 @@
 //   --alpha a (Yukawa decay / Gaussian width, default 1)
 //   --geometry cube|sphere|mesh  (default cube)   --seed s (points, default 0)
 //   --rhs reference|uniform  (b = A x_true with x_true ~ U(0,1), or b ~ U(-1,1); default reference)
-//   --rhs-seed s (default 0)  --leveled-rank r (default 0)  --mode h2|hss (default h2)
+//   --rhs-seed s (default 5489, the seed of a default-constructed std::mt19937_64 as in
+//     main.cpp:73-76, so x_true is the reference's)  --leveled-rank r (default 0)  --mode h2|hss (default h2)
 //   --prec-eps e (preconditioner truncation tolerance, default 0 = fixed rank)
 //   --factor fp64|fp32  --gmres reference|cgs2  --inner fp64|fp32
 //   --restart m (default 10)  --outer k (default 50)  --tol t (default eps)
@@ -42,7 +43,7 @@ struct Args {
               inner = "fp64";
   double diag = 1e3, alpha = 1., prec_eps = 0., tol = -1.;
   unsigned seed = 0;
-  unsigned long long rhs_seed = 0;
+  unsigned long long rhs_seed = 5489;
   long long leveled_rank = 0, restart = 10, outer = 50;
   const char* csv = nullptr;
   int device = 0;

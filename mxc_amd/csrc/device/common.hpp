@@ -156,6 +156,16 @@ struct mxg_ctx_s {
   };
   Timed timed[MXG_PROF_CLASSES];
   std::vector<hipEvent_t> event_pool;
+  // GMRES workspace, grown on demand and kept for later calls on the context
+  // (the FP64 Krylov bases of 8 right-hand sides at N = 2^20 and restart 30
+  // are 2 GB); calls on one context run one after another
+  struct GmresWork {
+    mxg::DevBuf<double> V, w, r, part, hd, yd, vcur;
+    mxg::DevBuf<float> V32;
+    mxg::DevBuf<int> kd;
+    double* hpin = nullptr;  // pinned host copy of one Arnoldi step
+    size_t hpin_n = 0;
+  } gmres;
 };
 
 namespace mxg {

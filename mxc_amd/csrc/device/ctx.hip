@@ -147,6 +147,8 @@ int mxg_finalize(mxg_ctx_t ctx) {
     MXG_HIP(hipSetDevice(ctx->device));
     drain(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->gmres.hpin) MXG_HIP(hipHostFree(ctx->gmres.hpin));
+    ctx->gmres.hpin = nullptr;
     MXG_HIP(hipStreamSynchronize(ctx->stream));
     MXG_HIP(hipStreamSynchronize(ctx->aux));
     MXG_HIP(hipEventDestroy(ctx->fork));

@@ -150,9 +150,17 @@ struct Gmres {
   int nblk;
   long long ldq, ldh;  // distance between right-hand sides: Krylov bases and per-step scalars
   bool f32;            // Krylov basis in FP32 (V32, with the newest column also in vcur)
-  DevBuf<double> V, w, r, part, hd, yd, vcur;
-  DevBuf<float> V32;
-  DevBuf<int> kd;
+  // workspace of the context (grown on demand, kept between calls)
+  struct Buf {
+    double* p;
+  };
+  Buf V, w, r, part, hd, yd, vcur;
+  struct Buf32 {
+    float* p;
+  } V32;
+  struct BufI {
+    int* p;
+  } kd;
   double* hpin = nullptr;  // pinned host copy of one Arnoldi step, ldh per right-hand side
 
   Gmres(mxg_h2_s* a, mxg_ulv_t mm, int restart, int nrhs, bool basis32)
@@ -160,22 +168,42 @@ struct Gmres {
     nblk = (int)ceil_div(n, DOT_ROWS);
     ldq = n * (long long)(m + 1);
     ldh = 2 * (m + 1) + 1;
+    auto& g = a->ctx->gmres;
+    // a buffer that grows may still be read by the previous call's kernels
+    MXG_HIP(hipStreamSynchronize(s));
+    V = {nullptr};
+    V32 = {nullptr};
+    vcur = {nullptr};
     if (f32) {
-      V32.alloc((size_t)ldq * nr);
-      vcur.alloc((size_t)n * nr);
+      g.V32.reserve((size_t)ldq * nr);
+      g.vcur.reserve((size_t)n * nr);
+      V32.p = g.V32.p;
+      vcur.p = g.vcur.p;
     } else {
-      V.alloc((size_t)ldq * nr);
+      g.V.reserve((size_t)ldq * nr);
+      V.p = g.V.p;
     }
-    w.alloc((size_t)n * nr);
-    r.alloc((size_t)n * nr);
-    part.alloc((size_t)nblk * (m + 1) * nr);
-    hd.alloc((size_t)ldh * nr);
-    yd.alloc((size_t)(m + 1) * nr);
-    kd.alloc(nr);
-    MXG_HIP(hipHostMalloc((void**)&hpin, (size_t)ldh * nr * sizeof(double), hipHostMallocDefault));
-  }
-  ~Gmres() {
-    if (hpin) (void)hipHostFree(hpin);
+    g.w.reserve((size_t)n * nr);
+    g.r.reserve((size_t)n * nr);
+    g.part.reserve((size_t)nblk * (m + 1) * nr);
+    g.hd.reserve((size_t)ldh * nr);
+    g.yd.reserve((size_t)(m + 1) * nr);
+    g.kd.reserve(nr);
+    w.p = g.w.p;
+    r.p = g.r.p;
+    part.p = g.part.p;
+    hd.p = g.hd.p;
+    yd.p = g.yd.p;
+    kd.p = g.kd.p;
+    const size_t hn = (size_t)ldh * nr;
+    if (g.hpin_n < hn) {
+      if (g.hpin) MXG_HIP(hipHostFree(g.hpin));
+      g.hpin = nullptr;
+      g.hpin_n = 0;
+      MXG_HIP(hipHostMalloc((void**)&g.hpin, hn * sizeof(double), hipHostMallocDefault));
+      g.hpin_n = hn;
+    }
+    hpin = g.hpin;
   }
   Gmres(const Gmres&) = delete;
   Gmres& operator=(const Gmres&) = delete;

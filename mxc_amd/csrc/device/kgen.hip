@@ -39,17 +39,22 @@ __global__ void __launch_bounds__(256) k_gen(const GenTask* __restrict__ tasks, 
 // evaluated once and applied to the NR right-hand sides of the launch.
 constexpr int DM_ROWS = 256, DM_CHUNK = 8192;
 
+// Rows row0 + blockIdx.x * DM_ROWS + t of the launch; partial sums of chunk c
+// and right-hand side q at part[(c * NR + q) * ldp + (row - row0)].
 template <int NR>
 __global__ void __launch_bounds__(256) k_dense_partial(const double* __restrict__ bodies, const double* __restrict__ x,
                                                        long long ldx, long long n, int kid, double diag, double alpha,
+                                                       long long row0, long long rows, long long ldp,
                                                        double* __restrict__ part) {
   __shared__ double sp[256 * 3];
   __shared__ double sx[NR][256];
-  long long i = (long long)blockIdx.x * DM_ROWS + threadIdx.x;
+  const long long li = (long long)blockIdx.x * DM_ROWS + threadIdx.x;
+  const long long i = row0 + li;
+  const bool live = li < rows && i < n;
   long long j0 = (long long)blockIdx.y * DM_CHUNK;
   long long j1 = min(j0 + DM_CHUNK, n);
   double p[3] = {0., 0., 0.};
-  if (i < n) { p[0] = bodies[3 * i]; p[1] = bodies[3 * i + 1]; p[2] = bodies[3 * i + 2]; }
+  if (live) { p[0] = bodies[3 * i]; p[1] = bodies[3 * i + 1]; p[2] = bodies[3 * i + 2]; }
   double acc[NR];
 #pragma unroll
   for (int q = 0; q < NR; q++) acc[q] = 0.;
@@ -71,19 +76,44 @@ __global__ void __launch_bounds__(256) k_dense_partial(const double* __restrict_
       for (int q = 0; q < NR; q++) acc[q] += kv * sx[q][c];
     }
   }
-  if (i < n)
+  if (live)
 #pragma unroll
-    for (int q = 0; q < NR; q++) part[((long long)blockIdx.y * NR + q) * n + i] = acc[q];
+    for (int q = 0; q < NR; q++) part[((long long)blockIdx.y * NR + q) * ldp + li] = acc[q];
 }
 
-__global__ void k_dense_reduce(const double* __restrict__ part, long long n, int nchunks, int nr, double* __restrict__ y,
-                               long long ldy) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// y[row0 + li] = sum over the chunks, in order, of the partial sums of the row
+__global__ void k_dense_reduce(const double* __restrict__ part, long long row0, long long rows, long long ldp, int nchunks,
+                               int nr, double* __restrict__ y, long long ldy) {
+  const long long li = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= rows) return;
   for (int q = 0; q < nr; q++) {
     double s = 0.;
-    for (int c = 0; c < nchunks; c++) s += part[((long long)c * nr + q) * n + i];
-    y[i + q * ldy] = s;
+    for (int c = 0; c < nchunks; c++) s += part[((long long)c * nr + q) * ldp + li];
+    y[row0 + li + q * ldy] = s;
+  }
+}
+
+// Y = K X for rows [row0, row0 + rows) of every right-hand side, partial sums
+// in part (ldp >= rows per chunk and right-hand side); each row is summed in
+// the same chunk order whatever the slab of rows it falls in.
+void dense_rows(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n, const double* d_X, long long ldx,
+                double* d_Y, long long ldy, long long nrhs, long long row0, long long rows, double* part, long long ldp) {
+  const int nchunks = (int)ceil_div(n, DM_CHUNK);
+  for (long long c0 = 0; c0 < nrhs;) {
+    const int nr = nrhs - c0 >= 8 ? 8 : nrhs - c0 >= 4 ? 4 : nrhs - c0 >= 2 ? 2 : 1;
+    const dim3 g(ceil_div(rows, DM_ROWS), nchunks), b(256);
+    const double* x = d_X + c0 * ldx;
+    switch (nr) {
+      case 8: hipLaunchKernelGGL(k_dense_partial<8>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, row0, rows, ldp, part); break;
+      case 4: hipLaunchKernelGGL(k_dense_partial<4>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, row0, rows, ldp, part); break;
+      case 2: hipLaunchKernelGGL(k_dense_partial<2>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, row0, rows, ldp, part); break;
+      default: hipLaunchKernelGGL(k_dense_partial<1>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, row0, rows, ldp, part); break;
+    }
+    MXG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_dense_reduce, dim3(ceil_div(rows, 256)), dim3(256), 0, ctx->stream, part, row0, rows, ldp, nchunks, nr,
+                       d_Y + c0 * ldy, ldy);
+    MXG_HIP(hipGetLastError());
+    c0 += nr;
   }
 }
 
@@ -145,26 +175,11 @@ int mxg_dense_matvec_batch(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies
     if (n == 0 || nrhs == 0) return;
     MXG_REQUIRE(d_bodies && d_X && d_Y, "null pointer");
     MXG_REQUIRE(ldx >= n && ldy >= n, "bad leading dimensions");
-    int nchunks = (int)ceil_div(n, DM_CHUNK);
+    const int nchunks = (int)ceil_div(n, DM_CHUNK);
     MXG_REQUIRE(ceil_div(n, DM_ROWS) < (1u << 31) && nchunks < 65536, "dense check size too large");
     DevBuf<double> part;
     part.alloc((size_t)nchunks * n * std::min<long long>(nrhs, 8));
-    for (long long c0 = 0; c0 < nrhs;) {
-      const int nr = nrhs - c0 >= 8 ? 8 : nrhs - c0 >= 4 ? 4 : nrhs - c0 >= 2 ? 2 : 1;
-      const dim3 g(ceil_div(n, DM_ROWS), nchunks), b(256);
-      const double* x = d_X + c0 * ldx;
-      switch (nr) {
-        case 8: hipLaunchKernelGGL(k_dense_partial<8>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, part.p); break;
-        case 4: hipLaunchKernelGGL(k_dense_partial<4>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, part.p); break;
-        case 2: hipLaunchKernelGGL(k_dense_partial<2>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, part.p); break;
-        default: hipLaunchKernelGGL(k_dense_partial<1>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, part.p); break;
-      }
-      MXG_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_dense_reduce, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, part.p, n, nchunks, nr,
-                         d_Y + c0 * ldy, ldy);
-      MXG_HIP(hipGetLastError());
-      c0 += nr;
-    }
+    dense_rows(ctx, k, d_bodies, n, d_X, ldx, d_Y, ldy, nrhs, 0, n, part.p, n);
     MXG_HIP(hipStreamSynchronize(ctx->stream));
   });
 }

@@ -28,7 +28,6 @@
 // sums of the terms per box in one more launch. At the leaf, the backward
 // sums write x_i to a buffer and the products U_i x_i run as their own list,
 // where a 256 x 256 bfloat16 factor is one wave's job.
-#include <chrono>
 #include <memory>
 
 #include "h2.hpp"
@@ -494,16 +493,26 @@ int mxg_ulv_create(mxg_h2_t h, int precision, mxg_ulv_t* out) {
     auto f = std::make_unique<mxg_ulv_s>();
     f->h = h;
     f->precision = precision;
-    auto t0 = std::chrono::steady_clock::now();
     if (precision == MXG_FP64)
       f->impl = std::make_unique<UlvImpl<double>>(h);
     else if (precision == MXG_FP32)
       f->impl = std::make_unique<UlvImpl<float>>(h);
     else
       f->impl = std::make_unique<UlvImpl<float, bf16_t>>(h);
+    // device time of the factorization proper (every level's launches and the
+    // host work that issues them), bracketed by events on the context stream
+    hipEvent_t e0, e1;
+    MXG_HIP(hipEventCreate(&e0));
+    MXG_HIP(hipEventCreate(&e1));
+    MXG_HIP(hipEventRecord(e0, h->ctx->stream));
     f->impl->factorize();
-    MXG_HIP(hipStreamSynchronize(h->ctx->stream));
-    f->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    MXG_HIP(hipEventRecord(e1, h->ctx->stream));
+    MXG_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    MXG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    f->seconds = 1e-3 * ms;
     *out = f.release();
   });
 }

@@ -914,6 +914,8 @@ orc_h2* orc_h2_create(const orc_problem* p, int kid, double diag, double alpha, 
 }
 
 void orc_h2_destroy(orc_h2* h) { delete h; }
+// threads of the matrix's later passes (matvec, factorization, substitution, GMRES)
+void orc_h2_set_threads(orc_h2* h, int nthreads) { h->nthreads = nthreads > 0 ? nthreads : 1; }
 long long orc_h2_nboxes(const orc_h2* h, long long level) { return h->lv[level].nb; }
 void orc_h2_dims(const orc_h2* h, long long level, long long* dims, long long* ranks) {
   const Level& Lv = h->lv[level];

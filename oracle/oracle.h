@@ -66,6 +66,7 @@ orc_h2* orc_h2_create(const orc_problem* p, int kid, double diag, double alpha, 
                       long long rank, long long leveled_rank, int orth, int hss_far, long long sampling,
                       int nthreads);
 void orc_h2_destroy(orc_h2* h);
+void orc_h2_set_threads(orc_h2* h, int nthreads);
 long long orc_h2_nboxes(const orc_h2* h, long long level);
 void orc_h2_dims(const orc_h2* h, long long level, long long* dims, long long* ranks);
 /* what: 0 = near blocks A, 1 = bases Q, 2 = R (V after orc_h2_factorize), 3 = couplings C */

@@ -52,6 +52,7 @@ def lib() -> C.CDLL:
             "orc_problem_bodies": (None, [P, P]),
             "orc_h2_create": (P, [P, c_int, c_d, c_d, c_d, c_d, c_ll, c_ll, c_int, c_int, c_ll, c_int]),
             "orc_h2_destroy": (None, [P]),
+            "orc_h2_set_threads": (None, [P, c_int]),
             "orc_h2_nboxes": (c_ll, [P, c_ll]),
             "orc_h2_dims": (None, [P, c_ll, P, P]),
             "orc_h2_near_count": (c_ll, [P, c_ll]),
@@ -183,6 +184,9 @@ class H2:
         ya, ye = np.zeros_like(x), np.zeros_like(x)
         lib().orc_h2_far_level(self.h, level, ptr(x), ptr(ya), ptr(ye))
         return ya, ye
+
+    def set_threads(self, n: int) -> None:
+        lib().orc_h2_set_threads(self.h, n)
 
     def factorize(self) -> int:
         return lib().orc_h2_factorize(self.h)
