@@ -55,6 +55,9 @@ int mxg_synchronize(mxg_ctx_t ctx);
 const char* mxg_last_error(void);
 /* HIP stream of the context (a hipStream_t) */
 void* mxg_stream(mxg_ctx_t ctx);
+/* free and total device memory of the context's GPU (hipMemGetInfo), for
+ * callers that admit work by its memory footprint */
+int mxg_mem_info(mxg_ctx_t ctx, size_t* free_bytes, size_t* total_bytes);
 
 /* device memory for callers without an allocator of their own */
 int mxg_malloc(mxg_ctx_t ctx, size_t bytes, void** d_ptr);
@@ -151,6 +154,18 @@ int mxg_h2_matvec(mxg_h2_t h, const double* d_x, double* d_y, long long nrhs, lo
 /* ---- ULV factorization: replaces compute_factorize / factorizeCopyNext / check_info
  *      (device_factorize.cu:56-142, h2matrix.cpp:346-446, device_precondition.cu:210-215) ---- */
 int mxg_ulv_create(mxg_h2_t h, int precision, mxg_ulv_t* f);
+/* Factorizations of count independent H^2 matrices on one context, whose
+ * trees may differ in depth, box sizes, ranks and near lists: sweep t factors
+ * level L_p - t of every matrix p that has it, and each phase of a sweep (the
+ * basis rotations, the rr inverses, V, the off-diagonal rotations, the Schur
+ * complements, the hand-over to the parent level) is one launch shared by all
+ * of them. out[count] receives one factorization per matrix, each equal to
+ * what mxg_ulv_create gives that matrix alone; mxg_ulv_factor_seconds of
+ * each is the time of the whole batch. The batched form of
+ * compute_factorize (device_factorize.cu:56-142), generalized from the
+ * reference's uniform padded blocks of one matrix to variable sizes across
+ * matrices. */
+int mxg_ulv_create_batch(const mxg_h2_t* h, long long count, int precision, mxg_ulv_t* out);
 int mxg_ulv_destroy(mxg_ulv_t f);
 /* info_per_level[levels+1]: number of boxes whose rr block had |det| <= DBL_MIN. */
 int mxg_ulv_info(mxg_ulv_t f, int* info_per_level);

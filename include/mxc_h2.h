@@ -80,6 +80,9 @@ int mxc_solver_counts(mxc_solver_t s, long long level, long long* counts);
 int mxc_solver_matvec(mxc_solver_t s, const double* d_x, double* d_y, int precision);
 /* info: levels + 1 ints (singular boxes per level) */
 int mxc_solver_factorize(mxc_solver_t s, int precision, int* info);
+/* ULV factorizations of count solvers on one context in shared launches
+ * (mxg_ulv_create_batch); info: levels + 1 ints per solver, concatenated */
+int mxc_solver_factorize_batch(mxc_solver_t* s, long long count, int precision, int* info);
 int mxc_solver_precondition(mxc_solver_t s, double* d_x);
 /* o->inner_operator may name the device matrix of another solver
  * (mxc_solver_device) used for the inner iterations. */

@@ -81,6 +81,7 @@ def gpu() -> C.CDLL:
             "mxg_finalize": [P],
             "mxg_device_count": [C.POINTER(c_int)],
             "mxg_synchronize": [P],
+            "mxg_mem_info": [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
             "mxg_malloc": [P, C.c_size_t, C.POINTER(P)],
             "mxg_free": [P, P],
             "mxg_memcpy_h2d": [P, P, P, C.c_size_t],
@@ -140,6 +141,7 @@ def h2() -> C.CDLL:
             "mxc_solver_counts": [P, c_ll, P],
             "mxc_solver_matvec": [P, P, P, c_int],
             "mxc_solver_factorize": [P, c_int, P],
+            "mxc_solver_factorize_batch": [P, c_ll, c_int, P],
             "mxc_solver_precondition": [P, P],
             "mxc_solver_gmres": [P, P, C.POINTER(MxgGmresOpts), P, P, P, C.POINTER(c_ll), C.POINTER(c_ll)],
             "mxc_solver_gmres_batch": [P, P, C.POINTER(MxgGmresOpts), c_ll, P, c_ll, P, c_ll, P, P, P],

@@ -161,6 +161,14 @@ int mxg_finalize(mxg_ctx_t ctx) {
 
 void* mxg_stream(mxg_ctx_t ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+int mxg_mem_info(mxg_ctx_t ctx, size_t* free_bytes, size_t* total_bytes) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx && free_bytes && total_bytes, "null argument");
+    MXG_HIP(hipSetDevice(ctx->device));
+    MXG_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  });
+}
+
 int mxg_synchronize(mxg_ctx_t ctx) {
   return guarded([&] {
     MXG_REQUIRE(ctx, "ctx is null");

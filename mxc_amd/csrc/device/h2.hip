@@ -31,13 +31,6 @@ namespace {
 // ---------------- small batched kernels ----------------
 
 template <class T>
-struct Copy2d {
-  const T* src;
-  T* dst;
-  int m, n, lds, ldd;
-};
-
-template <class T>
 __global__ void __launch_bounds__(256) k_copy2d(const Copy2d<T>* __restrict__ tasks, const long long* __restrict__ prefix,
                                                 int ntasks) {
   const long long blk = blockIdx.x;
@@ -46,7 +39,11 @@ __global__ void __launch_bounds__(256) k_copy2d(const Copy2d<T>* __restrict__ ta
   const long long e = (blk - prefix[t]) * 256 + threadIdx.x;
   if (e >= (long long)c.m * c.n) return;
   const int i = (int)(e % c.m), j = (int)(e / c.m);
-  c.dst[i + (long long)j * c.ldd] = c.src[i + (long long)j * c.lds];
+  const T v = c.src[i + (long long)j * c.lds];
+  if (c.trans)
+    c.dst[j + (long long)i * c.ldd] = v;
+  else
+    c.dst[i + (long long)j * c.ldd] = v;
 }
 
 template <class Td, class Ts>
@@ -155,14 +152,13 @@ std::vector<T> to_vec(const long long* p, long long n) {
 }  // namespace
 
 template <class T>
-void copy2d_batched(const std::vector<const T*>& src, const std::vector<T*>& dst, const std::vector<int>& m,
-                    const std::vector<int>& n, const std::vector<int>& lds, const std::vector<int>& ldd, hipStream_t s) {
+void copy2d_run(const std::vector<Copy2d<T>>& all, hipStream_t s) {
   std::vector<Copy2d<T>> tasks;
   std::vector<long long> prefix{0};
-  for (size_t t = 0; t < src.size(); t++) {
-    long long cnt = (long long)m[t] * n[t];
+  for (const Copy2d<T>& c : all) {
+    const long long cnt = (long long)c.m * c.n;
     if (cnt <= 0) continue;
-    tasks.push_back(Copy2d<T>{src[t], dst[t], m[t], n[t], lds[t], ldd[t]});
+    tasks.push_back(c);
     prefix.push_back(prefix.back() + (cnt + 255) / 256);
   }
   if (tasks.empty()) return;
@@ -174,6 +170,18 @@ void copy2d_batched(const std::vector<const T*>& src, const std::vector<T*>& dst
   hipLaunchKernelGGL(k_copy2d<T>, dim3((unsigned)prefix.back()), dim3(256), 0, s, dt.p, dp.p, (int)tasks.size());
   MXG_HIP(hipGetLastError());
   MXG_HIP(hipStreamSynchronize(s));
+}
+
+template void copy2d_run<double>(const std::vector<Copy2d<double>>&, hipStream_t);
+template void copy2d_run<float>(const std::vector<Copy2d<float>>&, hipStream_t);
+template void copy2d_run<int>(const std::vector<Copy2d<int>>&, hipStream_t);
+
+template <class T>
+void copy2d_batched(const std::vector<const T*>& src, const std::vector<T*>& dst, const std::vector<int>& m,
+                    const std::vector<int>& n, const std::vector<int>& lds, const std::vector<int>& ldd, hipStream_t s) {
+  std::vector<Copy2d<T>> tasks;
+  for (size_t t = 0; t < src.size(); t++) tasks.push_back(Copy2d<T>{src[t], dst[t], m[t], n[t], lds[t], ldd[t], 0, 0});
+  copy2d_run<T>(tasks, s);
 }
 
 template void copy2d_batched<double>(const std::vector<const double*>&, const std::vector<double*>&, const std::vector<int>&,

@@ -121,6 +121,17 @@ struct mxg_h2_s {
 };
 
 namespace mxg {
+// One block copy dst = src (m x n, column major), or dst = src^T with trans.
+template <class T>
+struct Copy2d {
+  const T* src;
+  T* dst;
+  int m, n, lds, ldd;
+  int trans, pad;
+};
+// every copy of the list in one launch
+template <class T>
+void copy2d_run(const std::vector<Copy2d<T>>& tasks, hipStream_t s);
 // launch helpers used by ulv.hip / gmres.hip
 template <class T>
 void copy2d_batched(const std::vector<const T*>& src, const std::vector<T*>& dst, const std::vector<int>& m,

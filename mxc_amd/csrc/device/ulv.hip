@@ -17,7 +17,8 @@
 //      complement onto the skeleton, no fill-in between non-neighbours)
 //   7  A_ij[:s, :s] -> the parent's near block (factorizeCopyNext,
 //      h2matrix.cpp:427-446)
-// Every step is one batched launch over the whole level.
+// Every step is one batched launch over the whole level; mxg_ulv_create_batch
+// shares each launch among the same level steps of several matrices.
 //
 // Substitution (h2matrix.cpp:448-537): forward L -> 0, x_i = V_i y_i,
 // z_i = x_i[:s] - sum_j A_ij.sr x_j[s_j:]; backward 0 -> L,
@@ -35,24 +36,123 @@
 namespace mxg {
 namespace {
 
-template <class T>
-__global__ void k_transpose_rows(const int* __restrict__ dims, const int* __restrict__ ranks, const long long* __restrict__ offQ,
-                                 const T* __restrict__ U, T* __restrict__ V) {
-  // V[:s, :] = U[:, :s]^T
-  const int i = blockIdx.x;
-  const int M = dims[i], s = ranks[i];
-  const T* Ui = U + offQ[i];
-  T* Vi = V + offQ[i];
-  for (long long e = threadIdx.x; e < (long long)s * M; e += blockDim.x) {
-    const int r = (int)(e % s), c = (int)(e / s);
-    Vi[r + (long long)c * M] = Ui[c + (long long)r * M];
-  }
-}
-
 // Scratch budget (elements) of one off-diagonal chunk of step 5.
 constexpr size_t PAIR_BUDGET = size_t(1) << 28;
 
 }  // namespace
+
+// GEMM tasks of one launch; segment indices are local to the list.
+template <class T>
+struct GemmList {
+  std::vector<GemmTask<T>> t;
+  std::vector<GemmSeg<T>> g;
+  int seg(const GemmSeg<T>& s) {
+    g.push_back(s);
+    return (int)g.size() - 1;
+  }
+  // a task over the segments [seg0, end of the list)
+  void task(T* C, int ldc, int m, int n, int seg0, double alpha, double beta) {
+    t.push_back(GemmTask<T>{C, ldc, m, n, seg0, (int)g.size() - seg0, alpha, beta});
+  }
+  void append(const GemmList& o) {
+    const int base = (int)g.size();
+    for (GemmTask<T> x : o.t) {
+      x.seg0 += base;
+      t.push_back(x);
+    }
+    g.insert(g.end(), o.g.begin(), o.g.end());
+  }
+  void run(hipStream_t s) const { gemm_batched<T>(t, g, s); }
+};
+
+// The launches of one level's factorization in their order. Plans of the
+// levels of several matrices merge phase by phase, so that one launch of each
+// phase serves all of them.
+template <class T>
+struct LevelPlan {
+  GemmList<T> rot1, rot2;                 // 1
+  std::vector<LuTask<T>> lu;              // 2
+  std::vector<Copy2d<T>> vtop;            // 3
+  GemmList<T> vbot;                       // 3
+  GemmList<T> rs, ss;                     // 4
+  std::vector<Copy2d<T>> rscopy;          // 4
+  std::vector<GemmList<T>> offa, offb;    // 5, per chunk of pairs
+  GemmList<T> sch1, sch2;                 // 6
+  std::vector<std::unique_ptr<DevBuf<T>>> scratch;  // held until the launches ran
+  T* alloc(size_t n) {
+    scratch.emplace_back(new DevBuf<T>());
+    scratch.back()->alloc(std::max<size_t>(n, 1));
+    return scratch.back()->p;
+  }
+  void merge(LevelPlan&& o) {
+    rot1.append(o.rot1);
+    rot2.append(o.rot2);
+    lu.insert(lu.end(), o.lu.begin(), o.lu.end());
+    vtop.insert(vtop.end(), o.vtop.begin(), o.vtop.end());
+    vbot.append(o.vbot);
+    rs.append(o.rs);
+    ss.append(o.ss);
+    rscopy.insert(rscopy.end(), o.rscopy.begin(), o.rscopy.end());
+    if (offa.size() < o.offa.size()) {
+      offa.resize(o.offa.size());
+      offb.resize(o.offb.size());
+    }
+    for (size_t c = 0; c < o.offa.size(); c++) {
+      offa[c].append(o.offa[c]);
+      offb[c].append(o.offb[c]);
+    }
+    sch1.append(o.sch1);
+    sch2.append(o.sch2);
+    for (auto& b : o.scratch) scratch.push_back(std::move(b));
+  }
+  void run(hipStream_t s) const {
+    rot1.run(s);
+    rot2.run(s);
+    lu_inverse_batched<T>(lu, s);
+    copy2d_run<T>(vtop, s);
+    vbot.run(s);
+    rs.run(s);
+    ss.run(s);
+    copy2d_run<T>(rscopy, s);
+    for (size_t c = 0; c < offa.size(); c++) {
+      offa[c].run(s);
+      offb[c].run(s);
+    }
+    sch1.run(s);
+    sch2.run(s);
+  }
+};
+
+template <class T, class S>
+struct UlvImpl;
+
+// Factorizes several H^2 matrices (same precision, same stream) level by
+// level from their leaves: sweep t factors level L_p - t of every matrix p
+// that has it, each phase of the sweep one launch over all of them, then
+// hands their skeleton blocks to the parent levels. The trees may differ in
+// depth, box sizes, ranks and near lists; every matrix gets the same results
+// as when factorized alone (the tasks are the same, only their launch is
+// shared).
+template <class T, class S>
+void factorize_batch(const std::vector<UlvImpl<T, S>*>& fs, hipStream_t s) {
+  long long depth = 0;
+  for (auto* f : fs) depth = std::max(depth, f->h->L);
+  for (long long t = 0; t <= depth; t++) {
+    LevelPlan<T> P;
+    std::vector<Copy2d<T>> next;
+    for (auto* f : fs) {
+      const long long l = f->h->L - t;
+      if (l < 0) continue;
+      LevelPlan<T> q;
+      f->plan_level(l, q);
+      P.merge(std::move(q));
+      if (l > 0) f->plan_copy_next(l, next);
+    }
+    P.run(s);
+    copy2d_run<T>(next, s);
+  }
+  for (auto* f : fs) f->finish();
+}
 
 struct UlvBase {
   virtual ~UlvBase() = default;
@@ -187,11 +287,11 @@ struct UlvImpl : UlvBase {
     MXG_HIP(hipStreamSynchronize(s));
   }
 
-  void factorize() override {
-    for (long long l = h->L; l >= 0; l--) {
-      factor_level(l);
-      if (l > 0) copy_next(l);
-    }
+  void factorize() override { factorize_batch<T, S>({this}, h->ctx->stream); }
+
+  // After every level is factored: the bfloat16 copies of the factors and the
+  // singular-block counts per level.
+  void finish() {
     hipStream_t s = h->ctx->stream;
     if constexpr (kCopy) {
       for (long long l = 0; l <= h->L; l++) {
@@ -210,8 +310,9 @@ struct UlvImpl : UlvBase {
     }
   }
 
-  void factor_level(long long l) {
-    hipStream_t s = h->ctx->stream;
+  // The launches of level l's factorization, appended to P (steps 1-6 of the
+  // header; P owns the scratch they use).
+  void plan_level(long long l, LevelPlan<T>& P) {
     Level& H = h->lv[l];
     Lv& F = lv[l];
     const long long nb = H.nb;
@@ -227,20 +328,15 @@ struct UlvImpl : UlvBase {
 
     // 1: two-sided rotation of the diagonal blocks
     {
-      DevBuf<T> T1;
-      T1.alloc(std::max<long long>(H.offQ[nb], 1));
-      std::vector<GemmTask<T>> t1, t2;
-      std::vector<GemmSeg<T>> s1, s2;
+      T* T1 = P.alloc(H.offQ[nb]);
       for (long long i = 0; i < nb; i++) {
         const int M = M_(i);
         if (M == 0) continue;
-        s1.push_back(GemmSeg<T>{Aij(H.diag[i]), Ui(i), M, M, M, 0, 0, 0, 0});
-        t1.push_back(GemmTask<T>{T1.p + H.offQ[i], M, M, M, (int)s1.size() - 1, 1, 1., 0.});
-        s2.push_back(GemmSeg<T>{Ui(i), T1.p + H.offQ[i], M, M, M, 1, 0, 0, 0});
-        t2.push_back(GemmTask<T>{Aij(H.diag[i]), M, M, M, (int)s2.size() - 1, 1, 1., 0.});
+        int g = P.rot1.seg(GemmSeg<T>{Aij(H.diag[i]), Ui(i), M, M, M, 0, 0, 0, 0});
+        P.rot1.task(T1 + H.offQ[i], M, M, M, g, 1., 0.);
+        g = P.rot2.seg(GemmSeg<T>{Ui(i), T1 + H.offQ[i], M, M, M, 1, 0, 0, 0});
+        P.rot2.task(Aij(H.diag[i]), M, M, M, g, 1., 0.);
       }
-      gemm_batched<T>(t1, s1, s);
-      gemm_batched<T>(t2, s2, s);
     }
     // 2: rr inverses
     {
@@ -249,60 +345,40 @@ struct UlvImpl : UlvBase {
         const long long r = M_(i) - S_(i);
         wsz += (size_t)r * r + 2 * r;
       }
-      DevBuf<T> work;
-      work.alloc(std::max<size_t>(wsz, 1));
-      std::vector<LuTask<T>> lu;
+      T* work = P.alloc(wsz);
       size_t wo = 0;
       for (long long i = 0; i < nb; i++) {
         const int M = M_(i), sI = S_(i), r = M - sI;
         if (r == 0) continue;
-        lu.push_back(LuTask<T>{Aij(H.diag[i]) + sI + (long long)sI * M, RRi(i), work.p + wo, F.info.p + i, r, M});
+        P.lu.push_back(LuTask<T>{Aij(H.diag[i]) + sI + (long long)sI * M, RRi(i), work + wo, F.info.p + i, r, M});
         wo += (size_t)r * r + 2 * r;
       }
-      lu_inverse_batched<T>(lu, s);
     }
-    // 3: V
-    {
-      hipLaunchKernelGGL(k_transpose_rows<T>, dim3((unsigned)nb), dim3(256), 0, s, H.d_dims.p, H.d_ranks.p, H.d_offQ.p, U, V);
-      MXG_HIP(hipGetLastError());
-      std::vector<GemmTask<T>> t;
-      std::vector<GemmSeg<T>> g;
-      for (long long i = 0; i < nb; i++) {
-        const int M = M_(i), sI = S_(i), r = M - sI;
-        if (r == 0) continue;
-        g.push_back(GemmSeg<T>{RRi(i), Ui(i) + (long long)sI * M, r, M, r, 0, 1, 0, 0});
-        t.push_back(GemmTask<T>{Vi(i) + sI, M, r, M, (int)g.size() - 1, 1, 1., 0.});
-      }
-      gemm_batched<T>(t, g, s);
+    // 3: V = [U[:, :s]^T ; rr^{-1} U[:, s:]^T]
+    for (long long i = 0; i < nb; i++) {
+      const int M = M_(i), sI = S_(i), r = M - sI;
+      if (sI > 0) P.vtop.push_back(Copy2d<T>{Ui(i), Vi(i), M, sI, M, M, 1, 0});
+      if (r == 0) continue;
+      const int g = P.vbot.seg(GemmSeg<T>{RRi(i), Ui(i) + (long long)sI * M, r, M, r, 0, 1, 0, 0});
+      P.vbot.task(Vi(i) + sI, M, r, M, g, 1., 0.);
     }
     // 4: rs <- rr^{-1} rs; ss -= sr rs
     {
       size_t tsz = 0;
       for (long long i = 0; i < nb; i++) tsz += (size_t)(M_(i) - S_(i)) * S_(i);
-      DevBuf<T> T2;
-      T2.alloc(std::max<size_t>(tsz, 1));
-      std::vector<GemmTask<T>> t1, t2;
-      std::vector<GemmSeg<T>> s1, s2;
-      std::vector<const T*> cs;
-      std::vector<T*> cd;
-      std::vector<int> cm, cn, cls, cld;
+      T* T2 = P.alloc(tsz);
       size_t to = 0;
       for (long long i = 0; i < nb; i++) {
         const int M = M_(i), sI = S_(i), r = M - sI;
         if (r == 0 || sI == 0) continue;
         T* Aii = Aij(H.diag[i]);
-        s1.push_back(GemmSeg<T>{RRi(i), Aii + sI, r, M, r, 0, 0, 0, 0});
-        t1.push_back(GemmTask<T>{T2.p + to, r, r, sI, (int)s1.size() - 1, 1, 1., 0.});
-        s2.push_back(GemmSeg<T>{Aii + (long long)sI * M, T2.p + to, M, r, r, 0, 0, 0, 0});
-        t2.push_back(GemmTask<T>{Aii, M, sI, sI, (int)s2.size() - 1, 1, -1., 1.});
-        cs.push_back(T2.p + to);
-        cd.push_back(Aii + sI);
-        cm.push_back(r); cn.push_back(sI); cls.push_back(r); cld.push_back(M);
+        int g = P.rs.seg(GemmSeg<T>{RRi(i), Aii + sI, r, M, r, 0, 0, 0, 0});
+        P.rs.task(T2 + to, r, r, sI, g, 1., 0.);
+        g = P.ss.seg(GemmSeg<T>{Aii + (long long)sI * M, T2 + to, M, r, r, 0, 0, 0, 0});
+        P.ss.task(Aii, M, sI, sI, g, -1., 1.);
+        P.rscopy.push_back(Copy2d<T>{T2 + to, Aii + sI, r, sI, r, M, 0, 0});
         to += (size_t)r * sI;
       }
-      gemm_batched<T>(t1, s1, s);
-      gemm_batched<T>(t2, s2, s);
-      copy2d_batched<T>(cs, cd, cm, cn, cls, cld, s);
     }
     // 5: off-diagonal rotations, in chunks of pairs
     {
@@ -313,7 +389,7 @@ struct UlvImpl : UlvBase {
       std::vector<long long> rowof(H.nrp.back());
       for (long long i = 0; i < nb; i++)
         for (long long ij = H.nrp[i]; ij < H.nrp[i + 1]; ij++) rowof[ij] = i;
-      size_t p0 = 0;
+      size_t p0 = 0, chunk = 0;
       while (p0 < pairs.size()) {
         size_t p1 = p0, need = 0;
         std::vector<size_t> off;
@@ -325,31 +401,33 @@ struct UlvImpl : UlvBase {
           need += e;
           p1++;
         }
-        DevBuf<T> P;
-        P.alloc(std::max<size_t>(need, 1));
-        std::vector<GemmTask<T>> ta, tb;
-        std::vector<GemmSeg<T>> sa, sb;
+        T* Pbuf = P.alloc(need);
+        if (P.offa.size() <= chunk) {
+          P.offa.resize(chunk + 1);
+          P.offb.resize(chunk + 1);
+        }
+        GemmList<T>& ga = P.offa[chunk];
+        GemmList<T>& gb = P.offb[chunk];
         for (size_t q = p0; q < p1; q++) {
           const long long ij = pairs[q], i = rowof[ij], j = H.ncol[ij];
           const int M = M_(i), N = M_(j), si = S_(i), sj = S_(j), ri = M - si;
-          T* Pq = P.p + off[q - p0];             // s_i x N
+          T* Pq = Pbuf + off[q - p0];            // s_i x N
           T* Qq = Pq + (size_t)si * N;           // M x s_j
           if (si > 0) {
-            sa.push_back(GemmSeg<T>{Ui(i), Aij(ij), M, M, M, 1, 0, 0, 0});
-            ta.push_back(GemmTask<T>{Pq, si, si, N, (int)sa.size() - 1, 1, 1., 0.});
-            sb.push_back(GemmSeg<T>{Pq, Ui(j), si, N, N, 0, 0, 0, 0});
-            tb.push_back(GemmTask<T>{Aij(ij), M, si, N, (int)sb.size() - 1, 1, 1., 0.});
+            int g = ga.seg(GemmSeg<T>{Ui(i), Aij(ij), M, M, M, 1, 0, 0, 0});
+            ga.task(Pq, si, si, N, g, 1., 0.);
+            g = gb.seg(GemmSeg<T>{Pq, Ui(j), si, N, N, 0, 0, 0, 0});
+            gb.task(Aij(ij), M, si, N, g, 1., 0.);
           }
           if (sj > 0 && ri > 0) {
-            sa.push_back(GemmSeg<T>{Aij(ij), Ui(j), M, N, N, 0, 0, 0, 0});
-            ta.push_back(GemmTask<T>{Qq, M, M, sj, (int)sa.size() - 1, 1, 1., 0.});
-            sb.push_back(GemmSeg<T>{Vi(i) + si, Qq, M, M, M, 0, 0, 0, 0});
-            tb.push_back(GemmTask<T>{Aij(ij) + si, M, ri, sj, (int)sb.size() - 1, 1, 1., 0.});
+            int g = ga.seg(GemmSeg<T>{Aij(ij), Ui(j), M, N, N, 0, 0, 0, 0});
+            ga.task(Qq, M, M, sj, g, 1., 0.);
+            g = gb.seg(GemmSeg<T>{Vi(i) + si, Qq, M, M, M, 0, 0, 0, 0});
+            gb.task(Aij(ij) + si, M, ri, sj, g, 1., 0.);
           }
         }
-        gemm_batched<T>(ta, sa, s);
-        gemm_batched<T>(tb, sb, s);
         p0 = p1;
+        chunk++;
       }
     }
     // 6: Schur complements onto the skeleton
@@ -358,10 +436,7 @@ struct UlvImpl : UlvBase {
       for (long long i = 0; i < nb; i++)
         for (long long ij = H.nrp[i]; ij < H.nrp[i + 1]; ij++)
           if (ij != H.diag[i]) t3 += (size_t)S_(i) * (M_(H.ncol[ij]) - S_(H.ncol[ij]));
-      DevBuf<T> T3;
-      T3.alloc(std::max<size_t>(t3, 1));
-      std::vector<GemmTask<T>> ta, tb;
-      std::vector<GemmSeg<T>> sa, sb;
+      T* T3 = P.alloc(t3);
       std::vector<size_t> t3off(H.nrp.back(), 0);
       size_t to = 0;
       for (long long i = 0; i < nb; i++) {
@@ -373,15 +448,15 @@ struct UlvImpl : UlvBase {
           const int N = M_(j), sj = S_(j), rj = N - sj;
           T* Aq = Aij(ij);
           if (si > 0 && sj > 0) {
-            const int seg0 = (int)sa.size();
-            if (ri > 0) sa.push_back(GemmSeg<T>{Aii + (long long)si * M, Aq + si, M, M, ri, 0, 0, 0, 0});
-            if (rj > 0) sa.push_back(GemmSeg<T>{Aq + (long long)sj * M, Aij(H.diag[j]) + sj, M, N, rj, 0, 0, 0, 0});
-            ta.push_back(GemmTask<T>{Aq, M, si, sj, seg0, (int)sa.size() - seg0, -1., 1.});
+            const int g = (int)P.sch1.g.size();
+            if (ri > 0) P.sch1.seg(GemmSeg<T>{Aii + (long long)si * M, Aq + si, M, M, ri, 0, 0, 0, 0});
+            if (rj > 0) P.sch1.seg(GemmSeg<T>{Aq + (long long)sj * M, Aij(H.diag[j]) + sj, M, N, rj, 0, 0, 0, 0});
+            P.sch1.task(Aq, M, si, sj, g, -1., 1.);
           }
           if (si > 0 && rj > 0) {
             t3off[ij] = to;
-            sa.push_back(GemmSeg<T>{Aq + (long long)sj * M, RRi(j), M, rj, rj, 0, 0, 0, 0});
-            ta.push_back(GemmTask<T>{T3.p + to, si, si, rj, (int)sa.size() - 1, 1, 1., 0.});
+            const int g = P.sch1.seg(GemmSeg<T>{Aq + (long long)sj * M, RRi(j), M, rj, rj, 0, 0, 0, 0});
+            P.sch1.task(T3 + to, si, si, rj, g, 1., 0.);
             to += (size_t)si * rj;
           }
         }
@@ -389,38 +464,32 @@ struct UlvImpl : UlvBase {
       for (long long i = 0; i < nb; i++) {
         const int M = M_(i), si = S_(i);
         if (si == 0) continue;
-        const int seg0 = (int)sb.size();
+        const int g = (int)P.sch2.g.size();
         for (long long ij = H.nrp[i]; ij < H.nrp[i + 1]; ij++) {
           if (ij == H.diag[i]) continue;
           const long long j = H.ncol[ij];
           const int sj = S_(j), rj = M_(j) - sj;
           if (rj == 0) continue;
-          sb.push_back(GemmSeg<T>{T3.p + t3off[ij], Aij(ij) + (long long)sj * M, si, M, rj, 0, 1, 0, 0});
+          P.sch2.seg(GemmSeg<T>{T3 + t3off[ij], Aij(ij) + (long long)sj * M, si, M, rj, 0, 1, 0, 0});
         }
-        if ((int)sb.size() > seg0) tb.push_back(GemmTask<T>{Aij(H.diag[i]), M, si, si, seg0, (int)sb.size() - seg0, -1., 1.});
+        if ((int)P.sch2.g.size() > g) P.sch2.task(Aij(H.diag[i]), M, si, si, g, -1., 1.);
       }
-      gemm_batched<T>(ta, sa, s);
-      gemm_batched<T>(tb, sb, s);
     }
   }
 
-  void copy_next(long long l) {
+  // 7: the skeleton blocks of level l's near pairs into the parent level's
+  // near blocks (factorizeCopyNext)
+  void plan_copy_next(long long l, std::vector<Copy2d<T>>& out) {
     Level& H = h->lv[l];
     Lv& F = lv[l];
-    Lv& P = lv[l - 1];
-    std::vector<const T*> cs;
-    std::vector<T*> cd;
-    std::vector<int> cm, cn, cls, cld;
+    Lv& Pl = lv[l - 1];
     for (long long i = 0; i < H.nb; i++)
       for (long long ij = H.nrp[i]; ij < H.nrp[i + 1]; ij++) {
         const long long j = H.ncol[ij];
         const int si = (int)H.ranks[i], sj = (int)H.ranks[j];
         if (si == 0 || sj == 0 || H.up_off[ij] < 0) continue;
-        cs.push_back(F.A.p + H.offA[ij]);
-        cd.push_back(P.A.p + H.up_off[ij]);
-        cm.push_back(si); cn.push_back(sj); cls.push_back((int)H.dims[i]); cld.push_back(H.up_ld[ij]);
+        out.push_back(Copy2d<T>{F.A.p + H.offA[ij], Pl.A.p + H.up_off[ij], si, sj, (int)H.dims[i], H.up_ld[ij], 0, 0});
       }
-    copy2d_batched<T>(cs, cd, cm, cn, cls, cld, h->ctx->stream);
   }
 
   void solve(double* x, long long ldx, long long nrhs) override {
@@ -514,6 +583,54 @@ int mxg_ulv_create(mxg_h2_t h, int precision, mxg_ulv_t* out) {
     (void)hipEventDestroy(e1);
     f->seconds = 1e-3 * ms;
     *out = f.release();
+  });
+}
+
+int mxg_ulv_create_batch(const mxg_h2_t* hs, long long count, int precision, mxg_ulv_t* out) {
+  return guarded([&] {
+    MXG_REQUIRE(hs && out && count >= 1, "null argument or empty batch");
+    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32 || precision == MXG_BF16, "unknown precision");
+    mxg_ctx_t ctx = hs[0] ? hs[0]->ctx : nullptr;
+    for (long long p = 0; p < count; p++) {
+      MXG_REQUIRE(hs[p], "null matrix in the batch");
+      MXG_REQUIRE(hs[p]->ctx == ctx, "every matrix of a batch must live on one context");
+      MXG_REQUIRE(hs[p]->flags & MXG_H2_FACTORABLE, "H2 matrix was created without MXG_H2_FACTORABLE");
+      for (long long l = 0; l <= hs[p]->L; l++) MXG_REQUIRE(hs[p]->lv[l].built, "H2 matrix not fully constructed");
+      for (long long q = 0; q < p; q++) MXG_REQUIRE(hs[q] != hs[p], "a matrix appears twice in the batch");
+    }
+    MXG_HIP(hipSetDevice(ctx->device));
+    std::vector<std::unique_ptr<mxg_ulv_s>> fs(count);
+    auto run = [&](auto* tag) {
+      using Impl = std::remove_pointer_t<decltype(tag)>;
+      std::vector<Impl*> impls;
+      for (long long p = 0; p < count; p++) {
+        fs[p] = std::make_unique<mxg_ulv_s>();
+        fs[p]->h = hs[p];
+        fs[p]->precision = precision;
+        auto impl = std::make_unique<Impl>(hs[p]);
+        impls.push_back(impl.get());
+        fs[p]->impl = std::move(impl);
+      }
+      hipEvent_t e0, e1;
+      MXG_HIP(hipEventCreate(&e0));
+      MXG_HIP(hipEventCreate(&e1));
+      MXG_HIP(hipEventRecord(e0, ctx->stream));
+      factorize_batch(impls, ctx->stream);
+      MXG_HIP(hipEventRecord(e1, ctx->stream));
+      MXG_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      MXG_HIP(hipEventElapsedTime(&ms, e0, e1));
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      for (auto& f : fs) f->seconds = 1e-3 * ms;
+    };
+    if (precision == MXG_FP64)
+      run((UlvImpl<double>*)nullptr);
+    else if (precision == MXG_FP32)
+      run((UlvImpl<float>*)nullptr);
+    else
+      run((UlvImpl<float, bf16_t>*)nullptr);
+    for (long long p = 0; p < count; p++) out[p] = fs[p].release();
   });
 }
 

@@ -249,6 +249,23 @@ int mxc_solver_factorize(mxc_solver_t s, int precision, int* info) {
   });
 }
 
+int mxc_solver_factorize_batch(mxc_solver_t* s, long long count, int precision, int* info) {
+  return guard([&] {
+    require(s && count >= 1, "null or empty batch");
+    std::vector<mxc::H2MatrixSolver*> ms;
+    for (long long p = 0; p < count; p++) {
+      require(s[p], "null solver in the batch");
+      ms.push_back(&s[p]->s);
+    }
+    mxc::H2MatrixSolver::factorizeBatch(ms, precision);
+    if (!info) return;
+    for (auto* m : ms) {
+      if (mxg_ulv_info(m->ulv, info) != MXG_OK) throw std::runtime_error(mxg_last_error());
+      info += m->levels + 1;
+    }
+  });
+}
+
 int mxc_solver_precondition(mxc_solver_t s, double* d_x) {
   return guard([&] {
     require(s, "null solver");

@@ -112,6 +112,26 @@ int H2MatrixSolver::factorizeM(int precision) {
   return singular;
 }
 
+int H2MatrixSolver::factorizeBatch(const std::vector<H2MatrixSolver*>& ms, int precision) {
+  std::vector<mxg_h2_t> hs;
+  for (H2MatrixSolver* m : ms) {
+    if (!m || m->levels < 0) throw std::runtime_error("factorizeBatch: matrix not constructed");
+    if (m->ulv) check(mxg_ulv_destroy(m->ulv));
+    m->ulv = nullptr;
+    hs.push_back(m->dev);
+  }
+  std::vector<mxg_ulv_t> fs(ms.size(), nullptr);
+  check(mxg_ulv_create_batch(hs.data(), (long long)hs.size(), precision, fs.data()));
+  int singular = 0;
+  for (size_t p = 0; p < ms.size(); p++) {
+    ms[p]->ulv = fs[p];
+    std::vector<int> info(ms[p]->levels + 1, 0);
+    check(mxg_ulv_info(fs[p], info.data()));
+    for (int v : info) singular += v ? 1 : 0;
+  }
+  return singular;
+}
+
 void H2MatrixSolver::solvePrecondition(double* X_dev) {
   if (levels < 0) return;
   if (!ulv) throw std::runtime_error("solvePrecondition before factorizeM");

@@ -50,6 +50,10 @@ class H2MatrixSolver {
   void matVecMul(double* X_dev);
   // ULV factorization (solver.cpp:90-100); returns the number of singular levels.
   int factorizeM(int precision = MXG_FP64);
+  // ULV factorizations of several independent matrices on one context, each
+  // phase of a level sweep one launch shared by all of them
+  // (mxg_ulv_create_batch); returns the number of singular levels in total.
+  static int factorizeBatch(const std::vector<H2MatrixSolver*>& ms, int precision = MXG_FP32);
   // x = M^{-1} x (solver.cpp:121-132).
   void solvePrecondition(double* X_dev);
   // GMRES (solver.cpp:144-201) with this matrix as the operator and M as the

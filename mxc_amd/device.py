@@ -88,6 +88,12 @@ class Context:
     def synchronize(self) -> None:
         N.check_gpu(N.gpu().mxg_synchronize(self.handle))
 
+    def mem_info(self):
+        """(free, total) bytes of device memory."""
+        f, t = C.c_size_t(), C.c_size_t()
+        N.check_gpu(N.gpu().mxg_mem_info(self.handle, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
     def close(self) -> None:
         if self.handle:
             for group in self._owned:

@@ -163,6 +163,20 @@ class H2:
         N.check_h2(N.h2().mxc_solver_factorize(self.handle, precision, info.ctypes.data))
         return info
 
+    @staticmethod
+    def factorize_batch(solvers, precision: int = N.MXG_FP32):
+        """ULV factorizations of several independent H^2 matrices on one
+        context, every phase of a level sweep one launch shared by all of them;
+        returns the singular-block counts per level of each."""
+        hs = (C.c_void_p * len(solvers))(*[s.handle for s in solvers])
+        info = np.zeros(sum(s.levels + 1 for s in solvers), np.int32)
+        N.check_h2(N.h2().mxc_solver_factorize_batch(hs, len(solvers), precision, info.ctypes.data))
+        out, o = [], 0
+        for s in solvers:
+            out.append(info[o:o + s.levels + 1])
+            o += s.levels + 1
+        return out
+
     def factor_seconds(self) -> float:
         return float(N.gpu().mxg_ulv_factor_seconds(N.h2().mxc_solver_ulv(self.handle)))
 
