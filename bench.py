@@ -53,6 +53,11 @@ def parse():
                     help="ULV of the mixed path: FP32 factorization with bfloat16-stored or FP32-stored factors")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=16384, help="points of the CPU baseline sample")
+    ap.add_argument("--workload", choices=["solve", "batch", "serve"], default="solve",
+                    help="solve: the metric (default); batch: 64 variable-size problems factorized in shared launches; "
+                         "serve: 16 resident problems under a Poisson request stream")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: offered load, requests per second")
+    ap.add_argument("--requests", type=int, default=160, help="serve: requests of the timed stream")
     ap.add_argument("--cpu-phases", action="store_true",
                     help="only time the oracle's phases on the host cores at N=20000 and N=65536 (no GPU work)")
     return ap.parse_args()
@@ -91,6 +96,10 @@ def main():
         print("bench.py: this build targets one GPU (the H^2 solve path keeps the whole tree on one device); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     from mxc_amd import _native as N
     from mxc_amd.device import Context, kernel_desc
@@ -259,6 +268,117 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.sampling, args.op_eps, n)
+    print(json.dumps(out))
+
+
+def bench_batch(args):
+    """64 independent problems (N log-uniform in [4096, 65536], theta in
+    {1, 2, 3}, leaf 128/256, rank in {32, 64, 128}, operator eps 1e-8, cube
+    Laplace): their FP32 ULV factorizations in shared launches against one
+    after another, then one FP64 GMRES solve (CGS2, restart 30) to 1e-8 per
+    problem, each checked with the dense FP64 product."""
+    from mxc_amd import _native as N
+    from mxc_amd.batch import Batch, log_uniform_specs
+    from mxc_amd.device import Context
+
+    ctx = Context(0)
+    specs = log_uniform_specs(64, seed=4)
+    t0 = time.time()
+    b = Batch(ctx, specs)
+    log(f"built {len(specs)} problems ({sum(s.n for s in specs)} points) in {time.time() - t0:.1f} s")
+    Ms = [m.M for m in b.members]
+    b.factorize(N.MXG_FP32)  # warm-up
+    reps = max(1, args.steps // 2)
+    batched = []
+    for _ in range(reps):
+        ctx.synchronize()
+        ts = time.perf_counter()
+        b.factorize(N.MXG_FP32)
+        batched.append((time.perf_counter() - ts, b.factor_seconds()))
+    single = []
+    for _ in range(reps):
+        ts = time.perf_counter()
+        dev = 0.
+        for M in Ms:
+            M.factorize(N.MXG_FP32)
+            dev += M.factor_seconds()
+        single.append((time.perf_counter() - ts, dev))
+    b.factorize(N.MXG_FP32)
+    res_max, dense_max, solve_s, inner = 0., 0., 0., []
+    for k, m in enumerate(b.members):
+        n = m.spec.n
+        rhs = np.random.default_rng(100 + k).uniform(-1, 1, n)
+        db, dx, dy = ctx.array(rhs), ctx.zeros(n), ctx.empty(n)
+        ts = time.perf_counter()
+        res, cyc, inn = m.A.gmres(m.M, db, dx, 1e-8, restart=30, max_cycles=20, mode=N.GMRES_CGS2)
+        solve_s += time.perf_counter() - ts
+        inner.append(inn)
+        m.problem.dense_matvec("laplace", 1e3, 0.0, dx, dy)
+        res_max = max(res_max, float(res[-1]))
+        dense_max = max(dense_max, float(np.linalg.norm(dy.download() - rhs) / np.linalg.norm(rhs)))
+        for a in (db, dx, dy):
+            a.free()
+    wall_b = min(t for t, _ in batched)
+    wall_s = min(t for t, _ in single)
+    out = {
+        "metric": "batched FP32 ULV factorization of 64 variable-size problems (shared launches)",
+        "value": round(len(specs) / wall_b, 3), "unit": "factorizations/s", "n_gpus": 1, "steps": reps,
+        "higher_is_better": True, "dtype": "f32 factorization, f64 GMRES", "data": "synthetic",
+        "config": {"workload": "64 problems, N log-uniform [4096, 65536], theta {1,2,3}, leaf {128,256}, "
+                               "rank {32,64,128}, operator eps 1e-8, cube Laplace K(0)=1e3",
+                   "points": int(sum(s.n for s in specs))},
+        "batched_wall_ms": round(1e3 * wall_b, 2), "batched_device_ms": round(1e3 * min(d for _, d in batched), 2),
+        "one_by_one_wall_ms": round(1e3 * wall_s, 2), "one_by_one_device_ms": round(1e3 * min(d for _, d in single), 2),
+        "speedup_wall": round(wall_s / wall_b, 3),
+        "gmres": {"tol": 1e-8, "solves": len(specs), "seconds": round(solve_s, 3), "inner_iters_max": int(max(inner)),
+                  "h2_relres_max": res_max, "dense_relres_max": dense_max},
+    }
+    b.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+def bench_serve(args):
+    """16 resident problems (N = 2^14 .. 2^19 in turn, cube Laplace K(0)=1e3,
+    theta 2, operator eps 1e-12, preconditioner eps 1e-6 with rank <= 64,
+    FP32 factorization with bfloat16-stored factors); a Poisson stream of
+    requests, each a problem id and 1..32 right-hand sides U(-1, 1), solved to
+    1e-10 by GMRES-IR; p50/p99 latency from arrival to solution at the offered
+    load, and solves/s."""
+    from mxc_amd.device import Context
+    from mxc_amd.serving import ProblemSpec, Server, latency_stats, poisson_load
+
+    sizes = [1 << (14 + (k % 6)) for k in range(16)]
+    ctx = Context(0)
+    free, total = ctx.mem_info()
+    ctx.close()
+    budget = int(0.6 * free)
+    srv = Server(hbm_budget=budget, slots=4, max_rhs=32)
+    try:
+        for pid, n in enumerate(sizes):
+            srv.register(pid, ProblemSpec(n=n, seed=500 + pid))
+        t0 = time.time()
+        warm = [srv.submit(pid, np.ones((n, 1))) for pid, n in enumerate(sizes)]
+        for r in warm:
+            r.wait(1200)
+        log(f"16 problems resident in {time.time() - t0:.1f} s, {srv.resident_bytes() / 2**30:.1f} GiB")
+        ncalls = len(srv.calls)
+        reqs = poisson_load(srv, list(range(16)), sizes, rate=args.rate, count=args.requests, seed=11)
+        st = latency_stats(reqs, srv.calls[ncalls:])
+        worst = max(max(r.resid) for r in reqs)
+        out = {
+            "metric": "serving: solves/s and p50/p99 request latency at a fixed offered load",
+            "value": round(st["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f32 ULV (bf16-stored factors) + f32 inner operator, f64 Arnoldi/residual", "data": "synthetic",
+            "config": {"workload": "16 resident problems N=2^14..2^19, cube Laplace, theta 2, tol 1e-10",
+                       "offered_load_req_per_s": args.rate, "requests": args.requests, "rhs_per_request": "1..32",
+                       "slots": 4, "max_rhs_per_call": 32, "hbm_budget_gib": round(budget / 2**30, 1)},
+            "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2), "mean_ms": round(st["mean_ms"], 2),
+            "latency_stats": st, "h2_relres_max": worst, "resident_gib": round(srv.resident_bytes() / 2**30, 2),
+            "evictions": srv.evictions,
+        }
+    finally:
+        srv.close()
     print(json.dumps(out))
 
 

@@ -28,7 +28,7 @@ class Spec:
     diag: float = 1e3
     alpha: float = 1.0
     geometry: str = "cube"
-    sampling: int = 2000
+    sampling: int = 0  # every far body, as the reference (problems of this size afford it)
 
 
 def log_uniform_specs(count: int, seed: int, n_lo: int = 4096, n_hi: int = 65536, thetas=(1.0, 2.0, 3.0),

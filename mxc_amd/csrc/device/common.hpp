@@ -163,6 +163,10 @@ struct mxg_ctx_s {
     mxg::DevBuf<double> V, w, r, part, hd, yd, vcur;
     mxg::DevBuf<float> V32;
     mxg::DevBuf<int> kd;
+    // Hessenberg matrices, least-squares scratch, rotations, rotated
+    // right-hand sides, thresholds and flags of the right-hand sides
+    mxg::DevBuf<double> H, W, cs, sn, gv, tb;
+    mxg::DevBuf<int> flags;
     double* hpin = nullptr;  // pinned host copy of one Arnoldi step
     size_t hpin_n = 0;
   } gmres;

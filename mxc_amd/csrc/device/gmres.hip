@@ -270,51 +270,122 @@ struct Gmres {
   }
 };
 
-// Householder least squares min ||s - H y|| for H ((k+1) x k, ld), as the
-// reference's H.householderQr().solve(s) (solver.cpp:190).
-void householder_ls(int k, int ld, std::vector<double> H, std::vector<double> s, std::vector<double>& y) {
+// ---- the small Arnoldi bookkeeping, on the device in FP64 ----
+// Per right-hand side q (one thread each): the Hessenberg matrix H_q
+// ((m+1) x m, column major), the Givens rotations cs_q / sn_q, the rotated
+// right-hand side g_q (m+1), the inner exit threshold tb_q = target * beta,
+// and the flags open_q (still extending its least-squares problem), k_q
+// (steps in this cycle) and inner_q (steps over all cycles).
+
+// Step i of every open right-hand side: column i of H_q from this step's
+// projections (h1 + h2 with CGS2) and |w|; with CGS2 the previous rotations
+// are applied to it, a new one zeroes its subdiagonal, g_q is rotated and the
+// inner exit tested (|g_{i+1}| <= target * beta); a zero |w| (happy breakdown)
+// closes the right-hand side as well.
+__global__ void k_arnoldi_step(int i, int m, int cgs2, int nr, const double* __restrict__ hd, long long ldh,
+                               double* __restrict__ H, double* __restrict__ cs, double* __restrict__ sn,
+                               double* __restrict__ gv, const double* __restrict__ tb, int* __restrict__ open,
+                               int* __restrict__ kq, int* __restrict__ inner) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nr || !open[q]) return;
+  const double* hp = hd + (long long)q * ldh;
+  double* col = H + (long long)q * (m + 1) * m + (long long)i * (m + 1);
+  for (int c = 0; c <= i; c++) col[c] = hp[c] + (cgs2 ? hp[(m + 1) + c] : 0.);
+  const double hn = sqrt(fmax(hp[2 * (m + 1)], 0.));
+  col[i + 1] = hn;
+  kq[q] = i + 1;
+  inner[q]++;
+  if (cgs2) {
+    double* c_ = cs + (long long)q * m;
+    double* s_ = sn + (long long)q * m;
+    double* g = gv + (long long)q * (m + 1);
+    for (int c = 0; c < i; c++) {
+      const double t = c_[c] * col[c] + s_[c] * col[c + 1];
+      col[c + 1] = -s_[c] * col[c] + c_[c] * col[c + 1];
+      col[c] = t;
+    }
+    const double a = col[i], b = col[i + 1];
+    const double rr = hypot(a, b);
+    c_[i] = rr > 0. ? a / rr : 1.;
+    s_[i] = rr > 0. ? b / rr : 0.;
+    col[i] = rr;
+    col[i + 1] = 0.;
+    g[i + 1] = -s_[i] * g[i];
+    g[i] = c_[i] * g[i];
+    if (fabs(g[i + 1]) <= tb[q]) open[q] = 0;
+  }
+  if (!(hn > 0.)) open[q] = 0;
+}
+
+// End of a cycle: y_q minimizing |beta_q e_1 - H_q y| over the k_q steps of
+// an active right-hand side (kd_q = k_q; 0 for an inactive one, which then
+// keeps its iterate). CGS2: back substitution with the rotated H_q and g_q.
+// Reference mode: Householder QR of H_q (its copy in W) and of beta e_1, then
+// back substitution, as the reference's H.householderQr().solve(s)
+// (solver.cpp:187-191).
+__global__ void k_cycle_lsq(int m, int cgs2, int nr, const double* __restrict__ H, const double* __restrict__ gv,
+                            const double* __restrict__ beta, const int* __restrict__ kq, const int* __restrict__ active,
+                            double* __restrict__ W, double* __restrict__ y, int* __restrict__ kd) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nr) return;
+  const int ld = m + 1;
+  double* yq = y + (long long)q * ld;
+  for (int t = 0; t <= m; t++) yq[t] = 0.;
+  const int k = active[q] ? kq[q] : 0;
+  kd[q] = k;
+  if (k == 0) return;
+  const double* Hq = H + (long long)q * ld * m;
+  if (cgs2) {
+    const double* g = gv + (long long)q * ld;
+    for (int i2 = k - 1; i2 >= 0; i2--) {
+      double v = g[i2];
+      for (int j = i2 + 1; j < k; j++) v -= Hq[i2 + (long long)j * ld] * yq[j];
+      yq[i2] = v / Hq[i2 + (long long)i2 * ld];
+    }
+    return;
+  }
+  double* A = W + (long long)q * ld * (m + 1);  // H_q copy, then the right-hand side in column m
+  for (int c = 0; c < k; c++)
+    for (int r = 0; r <= k; r++) A[r + (long long)c * ld] = Hq[r + (long long)c * ld];
+  double* sv = A + (long long)m * ld;
+  for (int r = 0; r <= k; r++) sv[r] = r == 0 ? beta[q] : 0.;
   const int rows = k + 1;
   for (int c = 0; c < k; c++) {
-    double* x = &H[c + (size_t)c * ld];
+    double* x = A + c + (long long)c * ld;
     const int len = rows - c;
     double tail = 0.;
     for (int i = 1; i < len; i++) tail += x[i] * x[i];
-    double tau = 0., beta = x[0];
+    double tau = 0., bet = x[0];
     if (tail > 2.2250738585072014e-308) {
-      beta = std::sqrt(x[0] * x[0] + tail);
-      if (x[0] >= 0.) beta = -beta;
-      const double sc = 1. / (x[0] - beta);
+      bet = sqrt(x[0] * x[0] + tail);
+      if (x[0] >= 0.) bet = -bet;
+      const double sc = 1. / (x[0] - bet);
       for (int i = 1; i < len; i++) x[i] *= sc;
-      tau = (beta - x[0]) / beta;
+      tau = (bet - x[0]) / bet;
     } else {
       for (int i = 1; i < len; i++) x[i] = 0.;
     }
-    x[0] = beta;
-    auto apply = [&](double* col) {
-      double wv = col[0];
-      for (int i = 1; i < len; i++) wv += x[i] * col[i];
+    x[0] = bet;
+    for (int j = c + 1; j <= k; j++) {
+      double* colj = (j < k) ? A + c + (long long)j * ld : sv + c;
+      double wv = colj[0];
+      for (int i = 1; i < len; i++) wv += x[i] * colj[i];
       wv *= tau;
-      col[0] -= wv;
-      for (int i = 1; i < len; i++) col[i] -= wv * x[i];
-    };
-    for (int j = c + 1; j < k; j++) apply(&H[c + (size_t)j * ld]);
-    apply(&s[c]);
+      colj[0] -= wv;
+      for (int i = 1; i < len; i++) colj[i] -= wv * x[i];
+    }
   }
-  y.assign(k, 0.);
   for (int i = k - 1; i >= 0; i--) {
-    double v = s[i];
-    for (int j = i + 1; j < k; j++) v -= H[i + (size_t)j * ld] * y[j];
-    y[i] = v / H[i + (size_t)i * ld];
+    double v = sv[i];
+    for (int j = i + 1; j < k; j++) v -= A[i + (long long)j * ld] * yq[j];
+    yq[i] = v / A[i + (long long)i * ld];
   }
 }
 
-// Per right-hand side state of the cycle loop.
+// Per right-hand side state kept on the host: the cycle bookkeeping.
 struct Rhs {
-  std::vector<double> H, gv, cs, sn, y;
   double beta = 0., normb = 1., target = 0.;
-  int k = 0;            // Arnoldi steps that enter this cycle's update
   bool active = false;  // takes part in the current cycle
-  bool open = false;    // still extending its least-squares problem in this cycle
   long long cycles = 0, inner = 0;
 };
 
@@ -327,12 +398,26 @@ void gmres_batch(mxg_h2_s* A, mxg_ulv_t M, const mxg_gmres_opts_t* o, int nr, co
   const long long R = o->max_cycles + 1;
   const bool ref = (o->mode == MXG_GMRES_REFERENCE);
   std::vector<Rhs> st(nr);
-  for (Rhs& q : st) {
-    q.H.assign((size_t)(m + 1) * m, 0.);
-    q.gv.assign(m + 1, 0.);
-    q.cs.assign(m, 0.);
-    q.sn.assign(m, 0.);
-  }
+  // device state of the small problems (context workspace)
+  auto& ws = A->ctx->gmres;
+  ws.H.reserve((size_t)(m + 1) * m * nr);
+  ws.W.reserve((size_t)(m + 1) * (m + 1) * nr);
+  ws.gv.reserve((size_t)(m + 1) * nr);
+  ws.cs.reserve((size_t)m * nr);
+  ws.sn.reserve((size_t)m * nr);
+  ws.tb.reserve((size_t)2 * nr);  // tb, then beta
+  ws.flags.reserve((size_t)4 * nr);  // open, k, inner, active
+  double* dH = ws.H.p;
+  double* dgv = ws.gv.p;
+  double* dtb = ws.tb.p;
+  double* dbeta = ws.tb.p + nr;
+  int* dopen = ws.flags.p;
+  int* dk = ws.flags.p + nr;
+  int* dinner = ws.flags.p + 2 * nr;
+  int* dactive = ws.flags.p + 3 * nr;
+  int* hflags = reinterpret_cast<int*>(g.hpin);  // 4 nr ints fit the pinned buffer (ldh >= 5 doubles per rhs)
+  const dim3 small((unsigned)ceil_div(nr, 64)), lanes(64);
+  MXG_HIP(hipMemsetAsync(dinner, 0, (size_t)nr * sizeof(int), g.s));
   // |b| and |x0|
   g.norms2(d_b, ldb);
   for (int q = 0; q < nr; q++) st[q].normb = g.hpin[q * g.ldh] > 0. ? std::sqrt(g.hpin[q * g.ldh]) : 1.;
@@ -352,8 +437,8 @@ void gmres_batch(mxg_h2_s* A, mxg_ulv_t M, const mxg_gmres_opts_t* o, int nr, co
     resid[q * R] = std::sqrt(std::max(g.hpin[q * g.ldh], 0.)) / st[q].normb;
     for (long long c = 1; c < R; c++) resid[q * R + c] = 0.;
   }
-  std::vector<int> kh(nr);
-  std::vector<double> yh((size_t)(m + 1) * nr);
+  std::vector<double> h_state((size_t)(m + 1) * nr + 2 * nr);
+  std::vector<int> h_open(2 * (size_t)nr);
   for (long long cyc = 0; cyc < o->max_cycles; cyc++) {
     bool any = false;
     for (int q = 0; q < nr; q++) {
@@ -363,28 +448,36 @@ void gmres_batch(mxg_h2_s* A, mxg_ulv_t M, const mxg_gmres_opts_t* o, int nr, co
     if (!any) break;
     g.precondition(g.r.p);
     g.norms2(g.r.p, n);
+    // cycle start: beta_q, the inner target, g_q = beta_q e_1, H_q = 0
+    double* hg = h_state.data();
+    double* htb = hg + (size_t)(m + 1) * nr;
+    double* hbeta = htb + nr;
+    std::fill(h_state.begin(), h_state.end(), 0.);
     for (int q = 0; q < nr; q++) {
       Rhs& S = st[q];
       S.beta = std::sqrt(std::max(g.hpin[q * g.ldh], 0.));
       if (!(S.beta > 0.)) S.active = false;
-      S.open = S.active;
-      S.k = 0;
-      std::fill(S.H.begin(), S.H.end(), 0.);
-      std::fill(S.gv.begin(), S.gv.end(), 0.);
-      S.gv[0] = S.beta;
       // inner target: reach tol on the true residual in this cycle, with margin
       S.target = S.active ? 0.3 * o->tol / resid[q * R + S.cycles] : 0.;
       if (o->inner_precision == MXG_FP32) S.target = std::max(S.target, 3e-6);
+      hg[(size_t)q * (m + 1)] = S.beta;
+      htb[q] = S.target * S.beta;
+      hbeta[q] = S.beta;
+      h_open[q] = S.active ? 1 : 0;
+      h_open[nr + q] = S.active ? 1 : 0;
     }
     any = false;
     for (int q = 0; q < nr; q++) any = any || st[q].active;
     if (!any) break;
+    MXG_HIP(hipMemcpyAsync(dgv, hg, (size_t)(m + 1) * nr * sizeof(double), hipMemcpyHostToDevice, g.s));
+    MXG_HIP(hipMemcpyAsync(dtb, htb, (size_t)2 * nr * sizeof(double), hipMemcpyHostToDevice, g.s));
+    MXG_HIP(hipMemcpyAsync(dopen, h_open.data(), (size_t)nr * sizeof(int), hipMemcpyHostToDevice, g.s));
+    MXG_HIP(hipMemcpyAsync(dactive, h_open.data() + nr, (size_t)nr * sizeof(int), hipMemcpyHostToDevice, g.s));
+    MXG_HIP(hipMemsetAsync(dk, 0, (size_t)nr * sizeof(int), g.s));
+    MXG_HIP(hipMemsetAsync(dH, 0, (size_t)(m + 1) * m * nr * sizeof(double), g.s));
     // V_q[:, 0] = r_q / beta_q (|r_q|^2 is still in hd)
     g.basis_set(0, g.r.p, n, g.hd.p);
     for (int i = 0; i < m; i++) {
-      bool open = false;
-      for (int q = 0; q < nr; q++) open = open || st[q].open;
-      if (!open) break;
       long long ldvi = 0;
       const double* vi = g.basis_col(i, &ldvi);
       double* h1d = g.hd.p;
@@ -400,65 +493,20 @@ void gmres_batch(mxg_h2_s* A, mxg_ulv_t M, const mxg_gmres_opts_t* o, int nr, co
       }
       g.dots_dev<double>(g.w.p, 1, n, g.w.p, n, nrd);
       g.basis_set(i + 1, g.w.p, n, nrd);
-      // one transfer per step: h, h2 and |w|^2 of every right-hand side
-      MXG_HIP(hipMemcpyAsync(g.hpin, g.hd.p, (size_t)g.ldh * nr * sizeof(double), hipMemcpyDeviceToHost, g.s));
+      hipLaunchKernelGGL(k_arnoldi_step, small, lanes, 0, g.s, i, m, ref ? 0 : 1, nr, g.hd.p, g.ldh, dH, ws.cs.p, ws.sn.p,
+                         dgv, dtb, dopen, dk, dinner);
+      MXG_HIP(hipGetLastError());
+      // one transfer per step: which right-hand sides still extend their basis
+      MXG_HIP(hipMemcpyAsync(hflags, dopen, (size_t)nr * sizeof(int), hipMemcpyDeviceToHost, g.s));
       MXG_HIP(hipStreamSynchronize(g.s));
-      for (int q = 0; q < nr; q++) {
-        Rhs& S = st[q];
-        if (!S.open) continue;
-        const double* hp = g.hpin + (size_t)q * g.ldh;
-        double* col = &S.H[(size_t)i * (m + 1)];
-        for (int c = 0; c <= i; c++) col[c] = hp[c] + (ref ? 0. : hp[(m + 1) + c]);
-        const double hn = std::sqrt(std::max(hp[2 * (m + 1)], 0.));
-        col[i + 1] = hn;
-        S.k = i + 1;
-        S.inner++;
-        if (!ref) {
-          // Givens update of column i and of the residual estimate
-          for (int c = 0; c < i; c++) {
-            const double t = S.cs[c] * col[c] + S.sn[c] * col[c + 1];
-            col[c + 1] = -S.sn[c] * col[c] + S.cs[c] * col[c + 1];
-            col[c] = t;
-          }
-          const double a = col[i], b = col[i + 1];
-          const double rr = std::hypot(a, b);
-          S.cs[i] = rr > 0. ? a / rr : 1.;
-          S.sn[i] = rr > 0. ? b / rr : 0.;
-          col[i] = rr;
-          col[i + 1] = 0.;
-          S.gv[i + 1] = -S.sn[i] * S.gv[i];
-          S.gv[i] = S.cs[i] * S.gv[i];
-          if (std::fabs(S.gv[i + 1]) <= S.target * S.beta) S.open = false;
-        }
-        if (!(hn > 0.)) S.open = false;  // happy breakdown
-      }
+      bool open = false;
+      for (int q = 0; q < nr; q++) open = open || hflags[q] != 0;
+      if (!open) break;
     }
     // least squares per right-hand side, then x_q += V_q y_q
-    for (int q = 0; q < nr; q++) {
-      Rhs& S = st[q];
-      kh[q] = S.active ? S.k : 0;
-      std::fill(yh.begin() + (size_t)q * (m + 1), yh.begin() + (size_t)(q + 1) * (m + 1), 0.);
-      if (!S.active || S.k == 0) continue;
-      const int k = S.k;
-      if (ref) {
-        std::vector<double> sv(k + 1, 0.);
-        sv[0] = S.beta;
-        std::vector<double> Hk((size_t)(k + 1) * k);
-        for (int c = 0; c < k; c++)
-          for (int rr = 0; rr <= k; rr++) Hk[rr + (size_t)c * (k + 1)] = S.H[rr + (size_t)c * (m + 1)];
-        householder_ls(k, k + 1, Hk, sv, S.y);
-      } else {
-        S.y.assign(k, 0.);
-        for (int i2 = k - 1; i2 >= 0; i2--) {
-          double v = S.gv[i2];
-          for (int j = i2 + 1; j < k; j++) v -= S.H[i2 + (size_t)j * (m + 1)] * S.y[j];
-          S.y[i2] = v / S.H[i2 + (size_t)i2 * (m + 1)];
-        }
-      }
-      std::copy(S.y.begin(), S.y.end(), yh.begin() + (size_t)q * (m + 1));
-    }
-    MXG_HIP(hipMemcpyAsync(g.yd.p, yh.data(), yh.size() * sizeof(double), hipMemcpyHostToDevice, g.s));
-    MXG_HIP(hipMemcpyAsync(g.kd.p, kh.data(), kh.size() * sizeof(int), hipMemcpyHostToDevice, g.s));
+    hipLaunchKernelGGL(k_cycle_lsq, small, lanes, 0, g.s, m, ref ? 0 : 1, nr, dH, dgv, dbeta, dk, dactive, ws.W.p, g.yd.p,
+                       g.kd.p);
+    MXG_HIP(hipGetLastError());
     g.basis_add(d_x, ldx);
     // true residual of every right-hand side
     g.matvec(d_x, ldx, g.r.p, MXG_FP64);
@@ -472,10 +520,11 @@ void gmres_batch(mxg_h2_s* A, mxg_ulv_t M, const mxg_gmres_opts_t* o, int nr, co
       resid[q * R + S.cycles] = std::sqrt(std::max(g.hpin[q * g.ldh], 0.)) / S.normb;
     }
   }
+  MXG_HIP(hipMemcpyAsync(hflags, dinner, (size_t)nr * sizeof(int), hipMemcpyDeviceToHost, g.s));
   MXG_HIP(hipStreamSynchronize(g.s));
   for (int q = 0; q < nr; q++) {
     if (iters_out) iters_out[q] = st[q].cycles;
-    if (inner_out) inner_out[q] = st[q].inner;
+    if (inner_out) inner_out[q] = hflags[q];
   }
 }
 
