@@ -16,12 +16,18 @@ every row in the same order whatever the batch width, and each right-hand
 side keeps its own Krylov basis and stopping test.
 
 Admission: a problem is built only when the resident footprint plus its
-estimated footprint fits the HBM budget; otherwise idle problems are evicted,
-least recently used first, and the request waits until memory frees up. The
-estimate is bytes per point of the problems built so far (a fixed guess before
-the first), and the footprint of a built problem is measured as the drop of
-free device memory over its construction, factorization and a first solve of
-max_rhs right-hand sides (which sizes its Krylov workspace).
+estimated footprint fits the HBM budget; otherwise residents that no slot
+holds are evicted (idle ones least recently used first, then those whose
+waiting requests are all younger than the one being admitted), and the
+request waits until memory frees up. Problems are taken up in the order of
+their oldest request, but one that cannot be admitted yet does not hold up
+resident problems behind it. The estimate is bytes per point of the problems
+built so far (a fixed guess before the first; with nothing resident a
+problem is built whatever the guess), and the footprint of a built problem is
+measured as the drop of free device memory over its construction,
+factorization and a first solve of max_rhs right-hand sides (which sizes its
+Krylov workspace); builds run one at a time and solves allocate nothing, so
+the drop is the problem's own.
 """
 from __future__ import annotations
 
@@ -132,7 +138,7 @@ class Resident:
 
 class Server:
     def __init__(self, hbm_budget: int, slots: int = 4, max_rhs: int = 32, tol: float = 1e-10, device: int = 0,
-                 bytes_per_point_guess: float = 400e3):
+                 bytes_per_point_guess: float = 400e3, resident_factory=None):
         self.budget = int(hbm_budget)
         self.max_rhs, self.tol, self.device = max_rhs, tol, device
         self.specs = {}
@@ -143,6 +149,7 @@ class Server:
         self.cv = threading.Condition()
         self.stop = False
         self.bpp_guess = bytes_per_point_guess
+        self.factory = resident_factory or Resident
         self.bpp_seen = 0.0
         self.evictions = 0
         self.calls = []  # (pid, nrhs, requests, t_start, t_end)
@@ -173,6 +180,16 @@ class Server:
         with self.cv:
             return sum(p.footprint for p in self.resident.values())
 
+    def state(self) -> str:
+        """One line: residents with their footprints, problems being built or
+        served, and the waiting requests per problem."""
+        with self.cv:
+            res = {pid: round(p.footprint / 2**20) for pid, p in self.resident.items()}
+            waiting = {pid: len(q) for pid, q in self.queues.items() if q}
+            return (f"budget {self.budget / 2**20:.0f} MiB, resident MiB {res}, building {sorted(self.building)}, "
+                    f"busy {sorted(self.busy)}, waiting {waiting}, estimate MiB/point "
+                    f"{(self.bpp_seen or self.bpp_guess) / 2**20:.4f}, evictions {self.evictions}")
+
     def close(self):
         with self.cv:
             self.stop = True
@@ -187,28 +204,54 @@ class Server:
     def _estimate(self, pid: int) -> int:
         return int((self.bpp_seen or self.bpp_guess) * self.specs[pid].n)
 
-    def _pick(self):
-        """The problem with the oldest waiting request that no slot holds."""
-        best, t = None, None
-        for pid, q in self.queues.items():
-            if q and pid not in self.busy and pid not in self.building:
-                if t is None or q[0].t_submit < t:
-                    best, t = pid, q[0].t_submit
-        return best
+    def _next(self):
+        """The problem to take up now, and whether it must be built first:
+        waiting problems in the order of their oldest request, the first that
+        is resident (and no slot holds), or that is not resident and can be
+        admitted (no other build running, and room made for it). A problem
+        that cannot be admitted yet does not hold up the resident problems
+        behind it, whose calls then free up memory."""
+        order = sorted((q[0].t_submit, pid) for pid, q in self.queues.items()
+                       if q and pid not in self.busy and pid not in self.building)
+        for t, pid in order:
+            if pid in self.resident:
+                return pid, False
+            if not self.building and self._make_room(self._estimate(pid), t):
+                return pid, True
+        return None, False
 
-    def _make_room(self, need: int) -> bool:
-        """Evict idle residents (least recently used first) until `need` more
-        bytes fit the budget; False when they cannot."""
+    def _make_room(self, need: int, t_oldest: float) -> bool:
+        """Evict residents until `need` more bytes fit the budget, and say
+        whether they fit. Candidates are the residents no slot holds that have
+        no waiting request (least recently used first), then those whose
+        waiting requests all arrived after t_oldest (latest first); nothing is
+        evicted unless enough can be. With nothing resident the problem is
+        admitted whatever the estimate, and one estimated above the whole budget
+        once every other resident is evicted: its build measures its real
+        footprint, and one that exceeds the budget is refused then."""
+        if not self.resident:
+            return True
+        need = min(need, self.budget)  # an estimate above the budget: admitted alone, then measured
         used = sum(p.footprint for p in self.resident.values())
-        idle = sorted((p for pid, p in self.resident.items() if pid not in self.busy and not self.queues[pid]),
-                      key=lambda p: p.last_used)
-        while used + need > self.budget and idle:
-            p = idle.pop(0)
+        if used + need <= self.budget:
+            return True
+        free = [p for pid, p in self.resident.items() if pid not in self.busy]
+        idle = sorted((p for p in free if not self.queues[p.pid]), key=lambda p: p.last_used)
+        later = sorted((p for p in free if self.queues[p.pid] and self.queues[p.pid][0].t_submit > t_oldest),
+                       key=lambda p: -self.queues[p.pid][0].t_submit)
+        victims, freed = [], 0
+        for p in idle + later:
+            if used - freed + need <= self.budget:
+                break
+            victims.append(p)
+            freed += p.footprint
+        if used - freed + need > self.budget:
+            return False
+        for p in victims:
             del self.resident[p.pid]
-            used -= p.footprint
             p.close()
             self.evictions += 1
-        return used + need <= self.budget
+        return True
 
     def _work(self):
         while True:
@@ -216,25 +259,25 @@ class Server:
                 while True:
                     if self.stop:
                         return
-                    pid = self._pick()
+                    pid, build = self._next()
                     if pid is not None:
-                        if pid in self.resident:
-                            break
-                        if not self.building and self._make_room(self._estimate(pid)):
-                            self.building.add(pid)
-                            break
+                        break
                     self.cv.wait(0.05)
-                if pid in self.building:
-                    build = True
+                if build:
+                    self.building.add(pid)
                 else:
-                    build = False
                     self.busy.add(pid)
             if build:
                 try:
-                    res = Resident(pid, self.specs[pid], self.device, self.max_rhs)
+                    res = self.factory(pid, self.specs[pid], self.device, self.max_rhs)
                     err = None
                 except Exception as e:  # noqa: BLE001 - the waiting requests receive the error
                     res, err = None, e
+                if res is not None and res.footprint > self.budget:
+                    fp = res.footprint
+                    res.close()
+                    res, err = None, MemoryError(f"problem {pid} needs {fp} bytes of device memory, more than the "
+                                                 f"budget of {self.budget}")
                 with self.cv:
                     self.building.discard(pid)
                     if res is not None:
@@ -278,10 +321,13 @@ class Server:
                 self.cv.notify_all()
 
 
-def poisson_load(server: Server, pids, sizes, rate: float, count: int, seed: int, max_rhs_per_request: int = 32):
+def poisson_load(server: Server, pids, sizes, rate: float, count: int, seed: int, max_rhs_per_request: int = 32,
+                 timeout: float = 120.0):
     """Submits `count` requests at Poisson arrivals of `rate` per second: each
     names a problem uniformly from pids and carries 1..max_rhs_per_request
-    right-hand sides uniform in [-1, 1]. Returns the requests (served)."""
+    right-hand sides uniform in [-1, 1]. Returns the requests (served); a
+    request not served within `timeout` seconds of the last arrival raises
+    TimeoutError with the server's state."""
     rng = np.random.default_rng(seed)
     reqs = []
     t = time.perf_counter()
@@ -295,7 +341,10 @@ def poisson_load(server: Server, pids, sizes, rate: float, count: int, seed: int
             time.sleep(max(0.0, min(t - time.perf_counter(), 0.01)))
         reqs.append(server.submit(pid, B))
     for r in reqs:
-        r.wait(timeout=600)
+        try:
+            r.wait(timeout=timeout)
+        except TimeoutError as e:
+            raise TimeoutError(f"{e}: {server.state()}") from None
     return reqs
 
 
