@@ -84,6 +84,16 @@ int mxg_dense_matvec(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long
  * is evaluated once per group of up to 8 columns. */
 int mxg_dense_matvec_batch(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n, const double* d_X,
                            long long ldx, double* d_Y, long long ldy, long long nrhs);
+/* mxg_dense_matvec_batch without any device allocation: no counterpart in the
+ * reference. For callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). The partial sums go to a
+ * 64 MiB scratch array that the library holds per GPU for the life of the
+ * process, in row slabs, so results equal mxg_dense_matvec_batch's bit for
+ * bit. n is limited to 2^28 points (MXG_ERR_ARG beyond). Calls that overlap
+ * on one GPU (from several threads or contexts) share the array and race:
+ * serialise them. */
+int mxg_dense_matvec_batch_noalloc(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n,
+                                   const double* d_X, long long ldx, double* d_Y, long long ldy, long long nrhs);
 
 /* ---- H^2 matrix: replaces createSpMatrixDesc/createDeviceCsr/createMatrixDesc and
  *      the numeric work of H2Matrix::construct (h2matrix.cpp:72-249). ---- */

@@ -17,6 +17,7 @@ c_int = C.c_int
 c_d = C.c_double
 P = C.c_void_p
 
+MXG_ERR_ARG = 1  # invalid argument or size beyond a documented limit
 MXG_FP64 = 0
 MXG_FP32 = 1
 MXG_BF16 = 2  # ULV only: FP32 factorization, bfloat16-stored factors for the substitution
@@ -92,6 +93,7 @@ def gpu() -> C.CDLL:
             "mxg_gen_block": [P, MxgKernel, c_ll, c_ll, P, P, P, c_ll],
             "mxg_dense_matvec": [P, MxgKernel, P, c_ll, P, P],
             "mxg_dense_matvec_batch": [P, MxgKernel, P, c_ll, P, c_ll, P, c_ll, c_ll],
+            "mxg_dense_matvec_batch_noalloc": [P, MxgKernel, P, c_ll, P, c_ll, P, c_ll, c_ll],
             "mxg_h2_create": [P, MxgKernel, P, c_ll, c_ll, c_int, C.POINTER(P)],
             "mxg_h2_destroy": [P],
             "mxg_h2_define_level": [P, c_ll, c_ll, P, P, P, P, P, P],

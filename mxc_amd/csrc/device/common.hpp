@@ -20,6 +20,9 @@ struct Error : std::runtime_error {
 };
 
 void set_last_error(const std::string& msg);
+// device address of the allocation-free dense check's scratch, looked up
+// once per device at context set-up (the device must be current)
+void dense_scratch_setup(int device);
 
 #define MXG_HIP(x)                                                                              \
   do {                                                                                          \

@@ -109,6 +109,16 @@ int mxg_init(int device, mxg_ctx_t* ctx) {
       delete c;
       throw Error(MXG_ERR_HIP, std::string("stream or event creation failed: ") + hipGetErrorString(e));
     }
+    try {
+      dense_scratch_setup(device);
+    } catch (...) {
+      (void)hipEventDestroy(c->join);
+      (void)hipEventDestroy(c->fork);
+      (void)hipStreamDestroy(c->aux);
+      (void)hipStreamDestroy(c->stream);
+      delete c;
+      throw;
+    }
     *ctx = c;
   });
 }

@@ -4,6 +4,8 @@
 // 64 x 32 tile of a column-major block; lane r owns a row (coalesced stores
 // down a column), the four waves take interleaved columns. Tasks of different
 // shapes share one launch through a prefix sum of their tile counts.
+#include <mutex>
+
 #include "internal.hpp"
 
 namespace mxg {
@@ -93,29 +95,64 @@ __global__ void k_dense_reduce(const double* __restrict__ part, long long row0, 
   }
 }
 
-// Y = K X for rows [row0, row0 + rows) of every right-hand side, partial sums
-// in part (ldp >= rows per chunk and right-hand side); each row is summed in
-// the same chunk order whatever the slab of rows it falls in.
-void dense_rows(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n, const double* d_X, long long ldx,
-                double* d_Y, long long ldy, long long nrhs, long long row0, long long rows, double* part, long long ldp) {
+// Y[:, 0:nr] = K X[:, 0:nr] for rows [row0, row0 + rows), nr in {1, 2, 4, 8},
+// partial sums in part (ldp >= rows per chunk and right-hand side); each row
+// is summed in the same chunk order whatever the slab of rows it falls in and
+// whatever nr is, so a column's result does not depend on either.
+void dense_group(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n, const double* x, long long ldx,
+                 double* y, long long ldy, int nr, long long row0, long long rows, double* part, long long ldp) {
   const int nchunks = (int)ceil_div(n, DM_CHUNK);
-  for (long long c0 = 0; c0 < nrhs;) {
-    const int nr = nrhs - c0 >= 8 ? 8 : nrhs - c0 >= 4 ? 4 : nrhs - c0 >= 2 ? 2 : 1;
-    const dim3 g(ceil_div(rows, DM_ROWS), nchunks), b(256);
-    const double* x = d_X + c0 * ldx;
-    switch (nr) {
-      case 8: hipLaunchKernelGGL(k_dense_partial<8>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, row0, rows, ldp, part); break;
-      case 4: hipLaunchKernelGGL(k_dense_partial<4>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, row0, rows, ldp, part); break;
-      case 2: hipLaunchKernelGGL(k_dense_partial<2>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, row0, rows, ldp, part); break;
-      default: hipLaunchKernelGGL(k_dense_partial<1>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, row0, rows, ldp, part); break;
-    }
-    MXG_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_dense_reduce, dim3(ceil_div(rows, 256)), dim3(256), 0, ctx->stream, part, row0, rows, ldp, nchunks, nr,
-                       d_Y + c0 * ldy, ldy);
-    MXG_HIP(hipGetLastError());
-    c0 += nr;
+  const dim3 g(ceil_div(rows, DM_ROWS), nchunks), b(256);
+  switch (nr) {
+    case 8: hipLaunchKernelGGL(k_dense_partial<8>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, row0, rows, ldp, part); break;
+    case 4: hipLaunchKernelGGL(k_dense_partial<4>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, row0, rows, ldp, part); break;
+    case 2: hipLaunchKernelGGL(k_dense_partial<2>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, row0, rows, ldp, part); break;
+    default: hipLaunchKernelGGL(k_dense_partial<1>, g, b, 0, ctx->stream, d_bodies, x, ldx, n, k.id, k.diag, k.alpha, row0, rows, ldp, part); break;
   }
+  MXG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_dense_reduce, dim3(ceil_div(rows, 256)), dim3(256), 0, ctx->stream, part, row0, rows, ldp, nchunks, nr,
+                     y, ldy);
+  MXG_HIP(hipGetLastError());
 }
+
+// widest group (8, 4, 2 or 1 columns) of the remaining ones
+inline int group_width(long long remaining, int cap) {
+  const int nr = remaining >= 8 ? 8 : remaining >= 4 ? 4 : remaining >= 2 ? 2 : 1;
+  return nr < cap ? nr : cap;
+}
+
+// ---- scratch of the allocation-free dense check ----
+// The partial sums of mxg_dense_matvec_batch_noalloc live in this array: one
+// copy per GPU, loaded with the code object and held until the process ends.
+// Host code never uses the array's name as an address: each device's address
+// is looked up once, at context set-up, and kept in scratch_ptr.
+constexpr long long DENSE_SCRATCH_DOUBLES = (64ll << 20) / (long long)sizeof(double);  // 64 MiB
+static __device__ __align__(256) double scratch[DENSE_SCRATCH_DOUBLES];
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double* scratch_ptr[MAX_DEVICES] = {};
+
+double* scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  return (device >= 0 && device < MAX_DEVICES) ? scratch_ptr[device] : nullptr;
+}
+
+}  // namespace
+
+// Looks up this device's copy of the scratch array (the device must be
+// current); called by mxg_init, once per device.
+void dense_scratch_setup(int device) {
+  MXG_REQUIRE(device >= 0 && device < MAX_DEVICES, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (scratch_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    throw Error(MXG_ERR_HIP, std::string("device address of the dense-check scratch: ") + hipGetErrorString(e));
+  scratch_ptr[device] = static_cast<double*>(p);
+}
+
+namespace {
 
 }  // namespace
 
@@ -179,7 +216,41 @@ int mxg_dense_matvec_batch(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies
     MXG_REQUIRE(ceil_div(n, DM_ROWS) < (1u << 31) && nchunks < 65536, "dense check size too large");
     DevBuf<double> part;
     part.alloc((size_t)nchunks * n * std::min<long long>(nrhs, 8));
-    dense_rows(ctx, k, d_bodies, n, d_X, ldx, d_Y, ldy, nrhs, 0, n, part.p, n);
+    for (long long c0 = 0; c0 < nrhs;) {
+      const int nr = group_width(nrhs - c0, 8);
+      dense_group(ctx, k, d_bodies, n, d_X + c0 * ldx, ldx, d_Y + c0 * ldy, ldy, nr, 0, n, part.p, n);
+      c0 += nr;
+    }
+    MXG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int mxg_dense_matvec_batch_noalloc(mxg_ctx_t ctx, mxg_kernel_t k, const double* d_bodies, long long n,
+                                   const double* d_X, long long ldx, double* d_Y, long long ldy, long long nrhs) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    MXG_REQUIRE(n >= 0 && nrhs >= 0, "negative size");
+    if (n == 0 || nrhs == 0) return;
+    MXG_REQUIRE(d_bodies && d_X && d_Y, "null pointer");
+    MXG_REQUIRE(ldx >= n && ldy >= n, "bad leading dimensions");
+    const long long nchunks = ceil_div(n, DM_CHUNK);
+    // one slab of DM_ROWS rows of one column must fit: n <= 2^28
+    MXG_REQUIRE(nchunks * DM_ROWS <= DENSE_SCRATCH_DOUBLES,
+                "n = " + std::to_string(n) + " exceeds the allocation-free dense check's limit of " +
+                    std::to_string(DENSE_SCRATCH_DOUBLES / DM_ROWS * DM_CHUNK) + " points (64 MiB of scratch)");
+    double* part = scratch_of(ctx->device);
+    MXG_REQUIRE(part, "the dense-check scratch of this device was not set up");
+    const int cap = (int)std::min<long long>(8, DENSE_SCRATCH_DOUBLES / (nchunks * DM_ROWS));
+    for (long long c0 = 0; c0 < nrhs;) {
+      const int nr = group_width(nrhs - c0, cap == 8 ? 8 : cap >= 4 ? 4 : cap >= 2 ? 2 : 1);
+      // rows per slab: as many whole DM_ROWS tiles as the scratch holds
+      const long long slab = std::min<long long>(DENSE_SCRATCH_DOUBLES / (nchunks * nr) / DM_ROWS * DM_ROWS, n);
+      for (long long row0 = 0; row0 < n; row0 += slab) {
+        const long long rows = std::min(slab, n - row0);
+        dense_group(ctx, k, d_bodies, n, d_X + c0 * ldx, ldx, d_Y + c0 * ldy, ldy, nr, row0, rows, part, slab);
+      }
+      c0 += nr;
+    }
     MXG_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
