@@ -232,6 +232,9 @@ def main():
             "workload": "H2 GMRES solve, unit-sphere Yukawa N=2^20, leaf 256",
             "N": n, "leaf": leaf, "theta": theta, "kernel": "yukawa exp(-r)/r, K(0)=1e3",
             "operator_eps": args.op_eps, "inner_operator_eps": args.inner_eps, "sampling": args.sampling,
+            "sampling_rule": "operators: the max(3, ceil(L/2)) nearest far shells of a box are kept whole, every "
+                             "farther shell keeps at most `sampling` bodies; preconditioner: `sampling` bodies in "
+                             "total over every outside body",
             "precond_eps": 1e-6, "precond_max_rank": 64,
             "restart": restart, "tol": tol, "levels": L, "rhs_per_step": args.batch,
         },
@@ -340,7 +343,7 @@ def bench_batch(args):
 
 def bench_serve(args):
     """16 resident problems (N = 2^14 .. 2^19 in turn, cube Laplace K(0)=1e3,
-    theta 2, operator eps 1e-12, preconditioner eps 1e-6 with rank <= 64,
+    theta 2, operator eps 1e-13, preconditioner eps 1e-6 with rank <= 64,
     FP32 factorization with bfloat16-stored factors); a Poisson stream of
     requests, each a problem id and 1..32 right-hand sides U(-1, 1), solved to
     1e-10 by GMRES-IR; p50/p99 latency from arrival to solution at the offered

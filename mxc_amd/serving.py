@@ -54,7 +54,8 @@ class ProblemSpec:
     diag: float = 1e3
     alpha: float = 1.0
     geometry: str = "cube"
-    op_eps: float = 1e-12       # operator: the residual of every GMRES cycle
+    op_eps: float = 1e-13       # operator: the residual of every GMRES cycle (1e-12 leaves up to 6e-11
+                                # of construction error on cube solutions at N = 65536)
     precond_eps: float = 1e-6   # preconditioner truncation, with a rank cap
     precond_rank: int = 64
     sampling: int = 2000
@@ -141,6 +142,10 @@ class Server:
                  bytes_per_point_guess: float = 400e3, resident_factory=None):
         self.budget = int(hbm_budget)
         self.max_rhs, self.tol, self.device = max_rhs, tol, device
+        # GMRES stops at half the target on the H^2 residual: the operator's
+        # construction error on a solution (about 1e-11 at eps 1e-12) takes the
+        # rest of the dense-check budget
+        self.h2_tol = 0.5 * tol
         self.specs = {}
         self.resident = {}
         self.queues = {}
@@ -302,7 +307,7 @@ class Server:
             for r in take:
                 r.t_start = t0
             try:
-                X, rr = res.solve(np.concatenate([r.B for r in take], axis=1), self.tol)
+                X, rr = res.solve(np.concatenate([r.B for r in take], axis=1), self.h2_tol)
                 o = 0
                 for r in take:
                     r.X, r.resid = X[:, o:o + r.nrhs], rr[o:o + r.nrhs]
