@@ -124,6 +124,10 @@ def main():
     t_in = time.time() - t0
     if A_in is not None:
         log(f"inner operator (eps {args.inner_eps:g}) built in {t_in:.1f} s")
+    # launch shapes of the matvecs the solves run, picked by the autotuner
+    tuning = {"fp64_operator": A.autotune(N.MXG_FP64, min(args.batch, 8)),
+              "fp32_operator": (A_in or A).autotune(N.MXG_FP32, min(args.batch, 8))}
+    log(f"autotuned matvec shapes: {tuning}")
     L = prob.levels
 
     bs = args.batch
@@ -167,7 +171,8 @@ def main():
         kern = {}
         import ctypes as C
         for cls, name in ((0, "matvec_near_fp64"), (1, "matvec_near_fp32"), (2, "ulv_fwd_leaf"), (3, "ulv_bwd_leaf"),
-                          (4, "matvec_upper"), (5, "matvec_far")):
+                          (4, "matvec_upper"), (5, "matvec_far"), (6, "matvec_fp64"), (7, "matvec_fp32"),
+                          (8, "ulv_solve")):
             cnt, ms = C.c_longlong(), C.c_double()
             N.check_gpu(N.gpu().mxg_profile_read(ctx.handle, cls, C.byref(cnt), C.byref(ms)))
             kern[name] = {"launches": cnt.value, "total_ms": ms.value,
@@ -185,6 +190,7 @@ def main():
         return dict(elapsed=elapsed, lat=lat, cycles=cycles, inner=inners, h2_resid=res, dense=dense, factor_s=fsec,
                     kernels=kern)
 
+    fflops = factor_flops(M, pts, L, theta)
     fp64 = run_path(N.MXG_FP64, N.MXG_FP64, sol["fp64"])
     mixed = run_path(N.MXG_BF16 if args.mixed_factor == "bf16" else N.MXG_FP32, N.MXG_FP32, sol["mixed"])
     # error of every mixed-path solution against the FP64 path's for the same right-hand side
@@ -252,7 +258,15 @@ def main():
             "rel_diff_vs_fp64_max": max(mix_err), "rel_diff_vs_fp64_median": float(np.median(mix_err)),
         },
         "setup_s": {"operator": round(t_op, 2), "preconditioner": round(t_pc, 2), "inner_operator": round(t_in, 2)},
+        "factorization": {
+            "algorithmic_flops": fflops,
+            "fp64": {"s": round(fp64["factor_s"], 4), "tflops": round(fflops / fp64["factor_s"] / 1e12, 2),
+                     "frac_of_fp64_mfma_peak": round(fflops / fp64["factor_s"] / 1e12 / 78.6, 3)},
+            "fp32_bf16": {"s": round(mixed["factor_s"], 4), "tflops": round(fflops / mixed["factor_s"] / 1e12, 2),
+                          "frac_of_fp32_mfma_peak": round(fflops / mixed["factor_s"] / 1e12 / 157.3, 3)},
+        },
         "kernels_mixed": mixed["kernels"],
+        "matvec_autotune": {k: {"ms": round(v[0], 4), **v[1]} for k, v in tuning.items()},
         "operator_levels": sizes,
         "roofline": {
             "kernel": f"k_pair_gemv<float, float, {nr}>: the matvec's launch of the leaf near products (FP32)",
@@ -383,6 +397,33 @@ def bench_serve(args):
     finally:
         srv.close()
     print(json.dumps(out))
+
+
+def factor_flops(M, points, levels, theta):
+    """Algorithmic flops of one ULV factorization of M, from its level
+    structure (ranks s_i, box sizes M_i, r_i = M_i - s_i, near pairs):
+    per box 4 M^3 (two-sided rotation) + 2 r^3 (LU and inverse of the rr
+    block) + 2 r^2 M (V) + 2 r^2 s + 2 s^2 r (rs and ss updates); per near
+    pair j != i, 2 M_i M_j (M_i + M_j) (two-sided rotation) and the single
+    Schur complement 2 s_i r_i s_j + 2 s_i r_j s_j + 2 s_i r_j^2 + 2 s_i^2 r_j."""
+    from mxc_amd.solver import build_tree, csr
+
+    _, child, body, R, Cc = build_tree(points, levels)
+    rp, col = csr(False, theta, levels, child, body, R, Cc)
+    total = 0.0
+    for l in range(levels + 1):
+        d, s = (a.astype(np.float64) for a in M.dims(l))
+        r = d - s
+        total += float(np.sum(4 * d**3 + 2 * r**3 + 2 * r**2 * d + 2 * r**2 * s + 2 * s**2 * r))
+        c0 = (1 << l) - 1
+        for i in range(1 << l):
+            for p in range(rp[c0 + i], rp[c0 + i + 1]):
+                j = col[p] - c0
+                if j == i:
+                    continue
+                total += 2 * d[i] * d[j] * (d[i] + d[j])
+                total += 2 * s[i] * r[i] * s[j] + 2 * s[i] * r[j] * s[j] + 2 * s[i] * r[j] ** 2 + 2 * s[i] ** 2 * r[j]
+    return total
 
 
 def host_info():

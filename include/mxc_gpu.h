@@ -155,6 +155,19 @@ int mxg_h2_define_level(mxg_h2_t h, long long level, long long nboxes, const lon
  * upload keep their own copies and are not affected. */
 int mxg_h2_upload(mxg_h2_t h, long long level, int what, const double* src, long long count);
 
+/* Launch-shape autotuner of the matvec (no counterpart in the reference): times
+ * every candidate shape of one precision's matvec for nrhs (1..8) right-hand
+ * sides, reps products each after a warm-up, and keeps the fastest for later
+ * matvecs. Candidates: the read-once symmetric products or the general ones
+ * for the leaf near blocks (matrices built from a kernel) and for the
+ * couplings (interpolative bases), and 16-, 32- or 64-row chunks for the
+ * upward products of levels with fewer than 1024 blocks. Every shape gives
+ * the same matvec up to rounding. best_ms: time per product of the kept shape. */
+int mxg_h2_autotune(mxg_h2_t h, int precision, long long nrhs, int reps, double* best_ms);
+/* The kept shape (near_sym / far_sym: 1 when the symmetric products run) and
+ * setting one by hand (split_rows: a multiple of 4 in 8..256). */
+int mxg_h2_get_tuning(mxg_h2_t h, int precision, int* near_sym, int* far_sym, int* split_rows);
+int mxg_h2_set_tuning(mxg_h2_t h, int precision, int near_sym, int far_sym, int split_rows);
 /* H^2 matvec y = A x on the device (matVecDeviceH2, device_csr_matrix.cu:328-338).
  * x, y: tree-ordered FP64 vectors of length nbodies, nrhs columns with leading
  * dimension ld. precision MXG_FP32 applies an FP32 copy of the operator data
@@ -200,6 +213,11 @@ typedef struct {
   mxg_h2_t inner_operator; /* NULL: A; else the H^2 matrix applied in the inner (Arnoldi)
                               iterations, e.g. A built at a looser tolerance, while the
                               residual of every cycle uses A (iterative refinement) */
+  int graph;           /* 1: with CGS2 and FP32 inner iterations, the inner steps of a call
+                          after its first replay one HIP graph of the step's operator and
+                          preconditioner launches, captured from that first step (their
+                          input is a fixed staging column); results are bit for bit those
+                          of plain launches. Ignored otherwise, or while profiling. */
 } mxg_gmres_opts_t;
 /* Solves A x = b with left preconditioner M. x holds the initial guess on entry
  * (a zero guess skips the first operator application: r = b).
@@ -230,8 +248,11 @@ enum {
   MXG_PROF_ULV_FWD_LEAF = 2,     /* forward substitution, leaf level (x = V y and z) */
   MXG_PROF_ULV_BWD_LEAF = 3,     /* backward substitution, leaf level */
   MXG_PROF_MATVEC_UPPER = 4,     /* upward products and per-box sums of every level */
-  MXG_PROF_MATVEC_FAR = 5,       /* coupling products of every level (one launch) */
-  MXG_PROF_CLASSES = 6
+  MXG_PROF_MATVEC_FAR = 5,       /* coupling products of every level */
+  MXG_PROF_MATVEC_FP64 = 6,      /* a whole FP64 matvec, fork to join */
+  MXG_PROF_MATVEC_FP32 = 7,      /* a whole FP32 matvec */
+  MXG_PROF_ULV_SOLVE = 8,        /* a whole substitution (every level, both sweeps) */
+  MXG_PROF_CLASSES = 9
 };
 int mxg_profile_enable(mxg_ctx_t ctx, int enable);
 int mxg_profile_reset(mxg_ctx_t ctx);

@@ -35,7 +35,7 @@ class MxgKernel(C.Structure):
 
 class MxgGmresOpts(C.Structure):
     _fields_ = [("tol", c_d), ("restart", c_int), ("max_cycles", c_int), ("mode", c_int), ("inner_precision", c_int),
-                ("inner_operator", C.c_void_p)]
+                ("inner_operator", C.c_void_p), ("graph", c_int)]
 
 
 class MxcH2Opts(C.Structure):
@@ -101,6 +101,9 @@ def gpu() -> C.CDLL:
             "mxg_ulv_create": [P, c_int, C.POINTER(P)],
             "mxg_ulv_destroy": [P],
             "mxg_h2_matvec": [P, P, P, c_ll, c_ll, c_int],
+            "mxg_h2_autotune": [P, c_int, c_ll, c_int, C.POINTER(c_d)],
+            "mxg_h2_get_tuning": [P, c_int, C.POINTER(c_int), C.POINTER(c_int), C.POINTER(c_int)],
+            "mxg_h2_set_tuning": [P, c_int, c_int, c_int, c_int],
             "mxg_h2_download": [P, c_ll, c_int, P, c_ll],
             "mxg_ulv_solve": [P, P, c_ll, c_ll],
             "mxg_ulv_info": [P, P],

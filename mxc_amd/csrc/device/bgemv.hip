@@ -532,9 +532,6 @@ constexpr size_t DYN_LDS_BYTES = 65536 - 2 * RED_BYTES;
 // reduction across waves is needed)
 constexpr long long WAVE_JOB_BYTES = 65536;
 
-// largest supported batch width not above nr
-inline int batch_width(int nr) { return nr >= 8 ? 8 : nr >= 4 ? 4 : nr >= 2 ? 2 : 1; }
-
 }  // namespace
 
 template <class T, class S>

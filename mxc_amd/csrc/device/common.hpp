@@ -184,6 +184,8 @@ struct ProfScope {
   hipStream_t st;
   hipEvent_t stop = nullptr;
   ProfScope(mxg_ctx_s* ctx, int k, hipStream_t s = nullptr);
-  ~ProfScope();
+  ~ProfScope() { end(); }
+  // records the stop event now (once; the destructor does it otherwise)
+  void end();
 };
 }  // namespace mxg

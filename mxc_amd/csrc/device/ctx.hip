@@ -69,8 +69,9 @@ ProfScope::ProfScope(mxg_ctx_s* ctx, int k, hipStream_t s) : c(ctx), cls(k), st(
   c->timed[cls].pending.emplace_back(start, stop);
 }
 
-ProfScope::~ProfScope() {
+void ProfScope::end() {
   if (stop) (void)hipEventRecord(stop, st);
+  stop = nullptr;
 }
 }  // namespace mxg
 

@@ -382,6 +382,39 @@ __global__ void k_cycle_lsq(int m, int cgs2, int nr, const double* __restrict__ 
   }
 }
 
+// One HIP graph of an inner step's launches (operator and preconditioner),
+// captured once per call on the context stream (the matvec's second stream
+// joins the capture through its fork/join events) and replayed for the later
+// steps. The graph holds the addresses of the call's buffers, so it never
+// outlives the call.
+struct StepGraph {
+  bool enabled = false, ready = false;
+  hipGraphExec_t exec = nullptr;
+  template <class F>
+  void capture(hipStream_t s, F launches) {
+    hipGraph_t graph = nullptr;
+    MXG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    try {
+      launches();
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    MXG_HIP(hipStreamEndCapture(s, &graph));
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    MXG_HIP(e);
+    ready = true;
+  }
+  StepGraph() = default;
+  StepGraph(const StepGraph&) = delete;
+  StepGraph& operator=(const StepGraph&) = delete;
+  ~StepGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
 // Per right-hand side state kept on the host: the cycle bookkeeping.
 struct Rhs {
   double beta = 0., normb = 1., target = 0.;
@@ -437,6 +470,10 @@ void gmres_batch(mxg_h2_s* A, mxg_ulv_t M, const mxg_gmres_opts_t* o, int nr, co
     resid[q * R] = std::sqrt(std::max(g.hpin[q * g.ldh], 0.)) / st[q].normb;
     for (long long c = 1; c < R; c++) resid[q * R + c] = 0.;
   }
+  // graph replay of the inner step: its input is the fixed staging column of
+  // the FP32 basis; events of the kernel timer cannot live in a graph
+  StepGraph step_graph;
+  step_graph.enabled = o->graph != 0 && g.f32 && !A->ctx->profiling;
   std::vector<double> h_state((size_t)(m + 1) * nr + 2 * nr);
   std::vector<int> h_open(2 * (size_t)nr);
   for (long long cyc = 0; cyc < o->max_cycles; cyc++) {
@@ -483,8 +520,18 @@ void gmres_batch(mxg_h2_s* A, mxg_ulv_t M, const mxg_gmres_opts_t* o, int nr, co
       double* h1d = g.hd.p;
       double* h2d = g.hd.p + (m + 1);
       double* nrd = g.hd.p + 2 * (m + 1);
-      g.matvec(vi, ldvi, g.w.p, o->inner_precision, Ain);
-      g.precondition(g.w.p);
+      if (step_graph.enabled && step_graph.ready) {
+        MXG_HIP(hipGraphLaunch(step_graph.exec, g.s));
+      } else {
+        // the first step of a graph-enabled call runs plainly (it finishes any
+        // set-up of the operator and factors) and then once more under capture
+        g.matvec(vi, ldvi, g.w.p, o->inner_precision, Ain);
+        g.precondition(g.w.p);
+        if (step_graph.enabled) step_graph.capture(g.s, [&] {
+          g.matvec(vi, ldvi, g.w.p, o->inner_precision, Ain);
+          g.precondition(g.w.p);
+        });
+      }
       g.basis_dots(i + 1, h1d);
       g.basis_sub(i + 1, h1d);
       if (!ref) {

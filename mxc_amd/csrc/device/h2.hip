@@ -381,6 +381,7 @@ void construct_basis(mxg_h2_s* h, Level& Lv, long long l, const long long* far_o
 void construct_coupling(mxg_h2_s* h, Level& Lv, int orth) {
   hipStream_t s = h->ctx->stream;
   const long long nb = Lv.nb;
+  Lv.sym_far = !orth;  // C = K(S_y, S_x) as generated, without the R factors
   if (Lv.C.n == 0) return;
   DevBuf<double> K1, K2;
   std::vector<GenTask> gens;
@@ -454,6 +455,35 @@ void basis_transpose(mxg_h2_s* h, Level& Lv) {
   MXG_HIP(hipStreamSynchronize(s));
 }
 
+// Position of the pair (j, i) in the level-local CSR (rp, col), whose rows
+// hold sorted columns; the lists of a matrix on one point set are symmetric.
+long long transposed_pos(const std::vector<long long>& rp, const std::vector<long long>& col, long long i,
+                         long long j) {
+  const long long* b = col.data() + rp[j];
+  const long long* e = col.data() + rp[j + 1];
+  const long long* f = std::lower_bound(b, e, i);
+  MXG_REQUIRE(f != e && *f == i, "block lists are not symmetric");
+  return rp[j] + (f - b);
+}
+
+// Far pairs i < j of a level with symmetric couplings as symmetric tasks over
+// the coupling blocks (offsets offC, leading dimension ld(i)) and the terms
+// (offsets toff).
+template <class LD>
+std::vector<SymTask> far_sym_tasks(const Level& Lv, const std::vector<long long>& offC, const std::vector<long long>& toff,
+                                   LD ld) {
+  std::vector<SymTask> out;
+  for (long long i = 0; i < Lv.nb; i++)
+    for (long long p = Lv.frp[i]; p < Lv.frp[i + 1]; p++) {
+      const long long j = Lv.fcol[p];
+      if (j <= i) continue;
+      const long long pt = transposed_pos(Lv.frp, Lv.fcol, i, j);
+      out.push_back(SymTask{offC[p], Lv.offZ[i], Lv.offZ[j], toff[p], toff[pt], (int)Lv.ranks[i], (int)Lv.ranks[j],
+                            (int)std::max<long long>(ld(i), 1), 0});
+    }
+  return out;
+}
+
 void build_matvec_lists(mxg_h2_s* h, Level& Lv, bool leaf) {
   hipStream_t s = h->ctx->stream;
   const long long nb = Lv.nb;
@@ -473,6 +503,8 @@ void build_matvec_lists(mxg_h2_s* h, Level& Lv, bool leaf) {
         Lv.far_t.push_back(PairTask{Lv.offC[p], Lv.offZ[j], Lv.toffF[p], (int)Lv.ranks[i], (int)Lv.ranks[j], (int)Lv.ranks[i], 0});
     }
   Lv.lenTF = Lv.toffF.back();
+  Lv.far_sym_t.clear();
+  if (Lv.sym_far) Lv.far_sym_t = far_sym_tasks(Lv, Lv.offC, Lv.toffF, [&](long long i) { return Lv.ranks[i]; });
   Lv.toffN.assign(Lv.nrp[nb] + 1, 0);
   Lv.lenTN = 0;
   if (leaf) {
@@ -484,6 +516,16 @@ void build_matvec_lists(mxg_h2_s* h, Level& Lv, bool leaf) {
           Lv.near_t.push_back(PairTask{Lv.offA[p], Lv.offX[j], Lv.toffN[p], (int)Lv.dims[i], (int)Lv.dims[j], (int)Lv.dims[i], 0});
       }
     Lv.lenTN = Lv.toffN.back();
+    // each near pair once (i <= j), with the term position of its transpose
+    Lv.near_sym_t.clear();
+    for (long long i = 0; i < nb; i++)
+      for (long long p = Lv.nrp[i]; p < Lv.nrp[i + 1]; p++) {
+        const long long j = Lv.ncol[p];
+        if (j < i) continue;
+        const long long pt = (j != i) ? Lv.toffN[transposed_pos(Lv.nrp, Lv.ncol, i, j)] : -1;
+        Lv.near_sym_t.push_back(SymTask{Lv.offA[p], Lv.offX[i], Lv.offX[j], Lv.toffN[p], pt, (int)Lv.dims[i],
+                                        (int)Lv.dims[j], (int)Lv.dims[i], 0});
+      }
   }
   Lv.d_toffF.upload(Lv.toffF, s);
   Lv.d_toffN.upload(Lv.toffN, s);
@@ -543,27 +585,42 @@ void ensure_matvec_jobs(mxg_h2_s* h) {
   DevBuf<T>& tb = Ops<T>::t(h);
   tb.alloc((size_t)h->tlen[pi] * MAX_RHS);
   if constexpr (pi == 0) h->x64.reserve((size_t)h->n * MAX_RHS);
+  const MatvecTuning& tn = h->tune[pi];
   std::vector<PairJob> prod;
+  std::vector<SymJob> prod_sym;
   for (long long l = 0; l <= L; l++) {
     Level& Lv = h->lv[l];
     std::vector<PairJob> up;
     const T* xin = (l == L) ? Ops<T>::x(h) : Ops<T>::Z(h, l + 1);
     const long long lzin = (l == L) ? h->n : h->lv[l + 1].lenZ;
     if (Lv.lenZ > 0) {
-      pair_jobs_append<T, T>(up, spread(Lv.up_t), Ops<T>::QT(Lv), (long long)Lv.offQT[Lv.nb], xin, Lv.lenX, lzin,
+      pair_jobs_append<T, T>(up, spread(Lv.up_t, SPLIT_BELOW, tn.split_rows), Ops<T>::QT(Lv), (long long)Lv.offQT[Lv.nb], xin, Lv.lenX, lzin,
                              Ops<T>::Z(h, l), Lv.lenZ, Lv.lenZ);
-      pair_jobs_append<T, T>(prod, pi ? Lv.far_t32 : Lv.far_t, Ops<T>::C(Lv), pi ? (long long)Lv.C32.n : (long long)Lv.C.n,
-                             Ops<T>::Z(h, l), Lv.lenZ, Lv.lenZ, tb.p + Lv.tbaseF[pi], pi ? Lv.lenTF32 : Lv.lenTF,
-                             h->tlen[pi]);
+      if (Lv.sym_far && tn.far_sym)
+        sym_jobs_append<T>(prod_sym, pi ? Lv.far_sym_t32 : Lv.far_sym_t, Ops<T>::C(Lv),
+                           pi ? (long long)Lv.C32.n : (long long)Lv.C.n, Ops<T>::Z(h, l), Lv.lenZ, Lv.lenZ,
+                           tb.p + Lv.tbaseF[pi], pi ? Lv.lenTF32 : Lv.lenTF, h->tlen[pi]);
+      else
+        pair_jobs_append<T, T>(prod, pi ? Lv.far_t32 : Lv.far_t, Ops<T>::C(Lv),
+                               pi ? (long long)Lv.C32.n : (long long)Lv.C.n, Ops<T>::Z(h, l), Lv.lenZ, Lv.lenZ,
+                               tb.p + Lv.tbaseF[pi], pi ? Lv.lenTF32 : Lv.lenTF, h->tlen[pi]);
     }
     pair_list_upload(Lv.up_pl[pi], up, s);
   }
   pair_list_upload(h->prod_pl[pi], prod, s);
+  sym_list_upload(h->prod_sym[pi], prod_sym, s);
   Level& Lf = h->lv[L];
   std::vector<PairJob> near;
-  pair_jobs_append<T, T>(near, Lf.near_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, h->n,
-                         tb.p + Lf.tbaseN[pi], Lf.lenTN, h->tlen[pi]);
+  const bool near_sym = h->sym_near && tn.near_sym;
+  if (!near_sym)
+    pair_jobs_append<T, T>(near, Lf.near_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, h->n,
+                           tb.p + Lf.tbaseN[pi], Lf.lenTN, h->tlen[pi]);
   pair_list_upload(h->near_pl[pi], near, s);
+  std::vector<SymJob> sym;
+  if (near_sym)
+    sym_jobs_append<T>(sym, Lf.near_sym_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, h->n,
+                       tb.p + Lf.tbaseN[pi], Lf.lenTN, h->tlen[pi]);
+  sym_list_upload(h->near_sym[pi], sym, s);
   MXG_HIP(hipStreamSynchronize(s));
   h->jobs_ready[pi] = true;
 }
@@ -602,10 +659,13 @@ void h2_matvec_levels(mxg_h2_s* h, T* y, long long ldy, int nrhs) {
   // the leaf near terms depend on x alone: they run on the second stream,
   // beside the upward pass and the far terms, which leave most of the GPU idle
   hipStream_t sa = h->ctx->aux;
+  ProfScope whole(h->ctx, f64 ? MXG_PROF_MATVEC_FP64 : MXG_PROF_MATVEC_FP32);
   MXG_HIP(hipEventRecord(h->ctx->fork, s));
   MXG_HIP(hipStreamWaitEvent(sa, h->ctx->fork, 0));
   {
     ProfScope prof(h->ctx, f64 ? MXG_PROF_MATVEC_LEAF_FP64 : MXG_PROF_MATVEC_LEAF_FP32, sa);
+    // one of the two lists is empty
+    sym_gemv<T>(h->near_sym[pi], nrhs, sa);
     pair_gemv<T, T>(h->near_pl[pi], nrhs, sa);
   }
   MXG_HIP(hipEventRecord(h->ctx->join, sa));
@@ -619,6 +679,7 @@ void h2_matvec_levels(mxg_h2_s* h, T* y, long long ldy, int nrhs) {
   {
     ProfScope prof(h->ctx, MXG_PROF_MATVEC_FAR);
     pair_gemv<T, T>(h->prod_pl[pi], nrhs, s);
+    sym_gemv<T>(h->prod_sym[pi], nrhs, s);
   }
   // downward pass: w = w_in + far terms, y = Q w (+ near terms at the leaf);
   // only the leaf level waits for the near terms
@@ -708,6 +769,9 @@ void pad_couplings32(Level& Lv, hipStream_t s) {
     }
   }
   Lv.lenTF32 = Lv.toffF32[np];
+  Lv.far_sym_t32.clear();
+  if (Lv.sym_far)
+    Lv.far_sym_t32 = far_sym_tasks(Lv, Lv.offC32, Lv.toffF32, [&](long long i) { return (Lv.ranks[i] + 3) / 4 * 4; });
   Lv.C32.alloc(std::max<long long>(Lv.offC32[np], 1));
   Lv.d_toffF32.upload(Lv.toffF32, s);
   if (pt.empty()) return;
@@ -876,6 +940,70 @@ void level_finish(mxg_h2_s* h, Level& Lv, bool leaf) {
 
 }  // namespace mxg
 
+namespace mxg {
+namespace {
+// Times every candidate launch shape of one precision's matvec for nrhs
+// right-hand sides (a warm-up, then reps timed products with HIP events) and
+// keeps the fastest; returns its time per product (ms).
+template <class T>
+double autotune(mxg_h2_s* h, int nrhs, int reps) {
+  constexpr int pi = sizeof(T) == 8 ? 0 : 1;
+  hipStream_t s = h->ctx->stream;
+  const long long n = h->n;
+  DevBuf<T> y;
+  y.alloc((size_t)n * nrhs);
+  bool any_far = false;
+  for (long long l = 0; l <= h->L; l++) any_far = any_far || h->lv[l].sym_far;
+  std::vector<MatvecTuning> cand;
+  for (int ns : {1, 0})
+    for (int fs : {1, 0})
+      for (int rows : {16, 32, 64}) {
+        if ((ns && !h->sym_near) || (fs && !any_far)) continue;
+        MatvecTuning t;
+        t.near_sym = ns;
+        t.far_sym = fs;
+        t.split_rows = rows;
+        cand.push_back(t);
+      }
+  hipEvent_t e0, e1;
+  MXG_HIP(hipEventCreate(&e0));
+  MXG_HIP(hipEventCreate(&e1));
+  double best = -1.;
+  MatvecTuning pick = h->tune[pi];
+  try {
+    for (const MatvecTuning& t : cand) {
+      h->tune[pi] = t;
+      h->jobs_ready[pi] = false;
+      ensure_matvec_jobs<T>(h);
+      MXG_HIP(hipMemsetAsync(Ops<T>::x(h), 0, (size_t)n * MAX_RHS * sizeof(T), s));
+      h2_matvec_levels<T>(h, y.p, n, nrhs);
+      MXG_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; r++) h2_matvec_levels<T>(h, y.p, n, nrhs);
+      MXG_HIP(hipEventRecord(e1, s));
+      MXG_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      MXG_HIP(hipEventElapsedTime(&ms, e0, e1));
+      const double per = (double)ms / reps;
+      if (best < 0. || per < best) {
+        best = per;
+        pick = t;
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->tune[pi] = pick;
+  h->jobs_ready[pi] = false;
+  ensure_matvec_jobs<T>(h);
+  return best;
+}
+}  // namespace
+}  // namespace mxg
+
 extern "C" {
 
 int mxg_h2_create(mxg_ctx_t ctx, mxg_kernel_t k, const double* bodies, long long nbodies, long long levels, int flags,
@@ -991,6 +1119,9 @@ int mxg_h2_define_level(mxg_h2_t h, long long l, long long nb, const long long* 
     Level& Lv = h->lv[l];
     Level* low = (l < h->L) ? &h->lv[l + 1] : nullptr;
     if (!low) {
+      // the caller's near blocks need not be symmetric
+      h->sym_near = false;
+      h->jobs_ready[0] = h->jobs_ready[1] = false;
       // the leaf's skeleton is its bodies in tree order, Q_in = I (h2matrix.cpp:179-188)
       upload_indices(h, Lv);
       DevBuf<long long> d_begin;
@@ -1063,11 +1194,67 @@ int mxg_h2_upload(mxg_h2_t h, long long l, int what, const double* src, long lon
       MXG_HIP(hipStreamSynchronize(s));
       return;
     }
+    if (what == 3 && Lv.sym_far) {
+      // the caller's couplings need not be symmetric
+      Lv.sym_far = false;
+      Lv.far_sym_t.clear();
+      Lv.far_sym_t32.clear();
+      h->jobs_ready[0] = h->jobs_ready[1] = false;
+    }
+    if (what == 0 && l == h->L && h->sym_near) {
+      // the caller's near blocks need not be symmetric: general products from now on
+      h->sym_near = false;
+      h->jobs_ready[0] = h->jobs_ready[1] = false;
+    }
     double* dst = what == 0 ? Lv.A.p : what == 1 ? Lv.Q.p : what == 2 ? Lv.R.p : Lv.C.p;
     MXG_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s));
     MXG_HIP(hipStreamSynchronize(s));
     if (what == 1) basis_transpose(h, Lv);
     h->shadow32 = false;  // FP32 copies are made again on their next use
+  });
+}
+
+
+int mxg_h2_autotune(mxg_h2_t h, int precision, long long nrhs, int reps, double* best_ms) {
+  return guarded([&] {
+    MXG_REQUIRE(h && best_ms, "null argument");
+    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32, "unknown precision");
+    MXG_REQUIRE(nrhs >= 1 && nrhs <= MAX_RHS && reps >= 1 && reps <= 1000, "nrhs must be 1..8 and reps 1..1000");
+    for (long long l = 0; l <= h->L; l++) MXG_REQUIRE(h->lv[l].built, "H2 matrix not fully constructed");
+    MXG_HIP(hipSetDevice(h->ctx->device));
+    if (precision == MXG_FP32) {
+      make_shadow32(h);
+      *best_ms = autotune<float>(h, (int)nrhs, reps);
+    } else {
+      *best_ms = autotune<double>(h, (int)nrhs, reps);
+    }
+    MXG_HIP(hipStreamSynchronize(h->ctx->stream));
+  });
+}
+
+int mxg_h2_get_tuning(mxg_h2_t h, int precision, int* near_sym, int* far_sym, int* split_rows) {
+  return guarded([&] {
+    MXG_REQUIRE(h && near_sym && far_sym && split_rows, "null argument");
+    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32, "unknown precision");
+    const MatvecTuning& t = h->tune[precision == MXG_FP64 ? 0 : 1];
+    *near_sym = t.near_sym && h->sym_near;
+    bool any_far = false;
+    for (long long l = 0; l <= h->L; l++) any_far = any_far || h->lv[l].sym_far;
+    *far_sym = t.far_sym && any_far;
+    *split_rows = t.split_rows;
+  });
+}
+
+int mxg_h2_set_tuning(mxg_h2_t h, int precision, int near_sym, int far_sym, int split_rows) {
+  return guarded([&] {
+    MXG_REQUIRE(h, "null argument");
+    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32, "unknown precision");
+    MXG_REQUIRE(split_rows >= 8 && split_rows <= 256 && split_rows % 4 == 0, "split_rows must be a multiple of 4 in 8..256");
+    const int pi = precision == MXG_FP64 ? 0 : 1;
+    h->tune[pi].near_sym = near_sym != 0;
+    h->tune[pi].far_sym = far_sym != 0;
+    h->tune[pi].split_rows = split_rows;
+    h->jobs_ready[pi] = false;
   });
 }
 

@@ -39,9 +39,19 @@ inline std::vector<PairTask> split_rows(const std::vector<PairTask>& tasks, int 
 // levels with fewer blocks than this get their rows split (chunks of 32)
 constexpr size_t SPLIT_BELOW = 1024;
 constexpr int SPLIT_ROWS = 32;
-inline std::vector<PairTask> spread(const std::vector<PairTask>& tasks) {
-  return tasks.size() < SPLIT_BELOW ? split_rows(tasks, SPLIT_ROWS) : tasks;
+inline std::vector<PairTask> spread(const std::vector<PairTask>& tasks, size_t below = SPLIT_BELOW,
+                                    int rows = SPLIT_ROWS) {
+  return tasks.size() < below ? split_rows(tasks, rows) : tasks;
 }
+
+// Launch shapes of one precision's matvec, chosen by mxg_h2_autotune: the
+// symmetric (read-once) or the general block products for the leaf near
+// blocks and for the couplings, and the row chunk of the upward products of
+// levels with fewer than 1024 blocks.
+struct MatvecTuning {
+  int near_sym = 0, far_sym = 0;
+  int split_rows = SPLIT_ROWS;
+};
 
 // The device form of a task: absolute addresses, and vec = 1 when the block
 // admits 16-byte column loads (address, rows and ld multiples of 16 bytes).
@@ -56,6 +66,34 @@ struct PairJob {
 };
 // right-hand sides one launch of the solve-phase kernels carries at most
 constexpr int MAX_RHS = 8;
+// right-hand sides of one launch out of nr remaining: 8, 4, 2 or 1
+inline int batch_width(int nr) { return nr >= 8 ? 8 : nr >= 4 ? 4 : nr >= 2 ? 2 : 1; }
+
+// A stored near block A_ij (i <= j) of a symmetric kernel matrix and the two
+// terms it gives: o1 = A_ij x_j (n rows) and, for j != i, o2 = A_ij^T x_i =
+// A_ji x_i (m rows; -1 for a diagonal block). Offsets into the matrix, the
+// input vector (xi, xj) and the term array of the pass.
+struct SymTask {
+  long long b, xi, xj, o1, o2;
+  int n, m, ld;
+  int pad;
+};
+// Device form: absolute addresses; out2 = nullptr for a diagonal block; vec =
+// 1 when the block admits 16-byte column loads.
+struct SymJob {
+  const void* B;
+  const void* xi;
+  const void* xj;
+  void* out1;
+  void* out2;
+  long long ldv, ldo;  // strides between right-hand sides of x and of the terms
+  int n, m, ld;
+  int vec;
+};
+struct SymList {
+  DevBuf<SymJob> jobs;
+  long long count = 0;
+};
 // Jobs of one launch: the workgroup-sized ones first, then the wave-sized.
 struct PairList {
   DevBuf<PairJob> jobs;
@@ -88,6 +126,13 @@ struct Level {
   // far terms have the padded row counts of C32: far_t32, toffF32)
   std::vector<long long> offQT, toffF, toffN, toffF32;
   std::vector<PairTask> up_t, far_t, near_t, far_t32;
+  // leaf near blocks with i <= j, each giving both of its transposed terms
+  std::vector<SymTask> near_sym_t;
+  // couplings generated as K(S_i, S_j) on the skeleton points (interpolative
+  // bases), so C_ji = C_ij^T bit for bit: each far pair i < j once, FP64 and
+  // padded FP32 (offC32 / toffF32)
+  bool sym_far = false;
+  std::vector<SymTask> far_sym_t, far_sym_t32;
   DevBuf<double> QT;
   DevBuf<float> QT32;
   PairList up_pl[2];  // [0] FP64, [1] FP32
@@ -116,6 +161,13 @@ struct mxg_h2_s {
   mxg::DevBuf<float> t32;
   mxg::PairList prod_pl[2];  // coupling products of every level
   mxg::PairList near_pl[2];  // leaf near products
+  // the leaf near blocks come from the kernel on one point set (A_ji = A_ij^T
+  // bit for bit): the near products read each stored pair once
+  // (near_sym); false once near data are defined or uploaded by the caller
+  bool sym_near = true;
+  mxg::SymList near_sym[2];
+  mxg::SymList prod_sym[2];  // coupling products of the levels with sym_far
+  mxg::MatvecTuning tune[2];  // [0] FP64, [1] FP32
   bool jobs_ready[2] = {false, false};
   long long tlen[2] = {1, 1};  // term count of the matvec per precision (the stride between right-hand sides)
 };
@@ -164,6 +216,16 @@ void pair_list_upload(PairList& pl, const std::vector<PairJob>& jobs, hipStream_
 // stored as S, arithmetic in T
 template <class T, class S>
 void pair_gemv(const PairList& pl, int nrhs, hipStream_t s);
+// Symmetric near products (symv.hip): jobs of tasks over the matrix B (nB
+// elements), the input x (nx per right-hand side, ldv apart) and the terms
+// (nout per right-hand side, ldo apart), checked against those sizes; and the
+// launch for nrhs <= MAX_RHS right-hand sides.
+template <class T>
+void sym_jobs_append(std::vector<SymJob>& jobs, const std::vector<SymTask>& tasks, const T* B, long long nB, const T* x,
+                     long long nx, long long ldv, T* out, long long nout, long long ldo);
+void sym_list_upload(SymList& sl, const std::vector<SymJob>& jobs, hipStream_t s);
+template <class T>
+void sym_gemv(const SymList& sl, int nrhs, hipStream_t s);
 // FP32 -> bfloat16, round to nearest even
 void to_bf16(bf16_t* dst, const float* src, size_t n, hipStream_t s);
 // w_i = w_in_i + sum_far tF, y_i = Q_i[:, :k] w_i (+ sum_near tN when tN !=

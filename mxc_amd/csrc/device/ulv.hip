@@ -499,6 +499,7 @@ struct UlvImpl : UlvBase {
     for (long long c0 = 0; c0 < nrhs; c0 += MAX_RHS) {
       const int nr = (int)std::min<long long>(MAX_RHS, nrhs - c0);
       convert2d(v, n, x + c0 * ldx, ldx, n, nr, s);
+      ProfScope whole(h->ctx, MXG_PROF_ULV_SOLVE);
       for (long long l = L; l >= 0; l--) {
         Level& H = h->lv[l];
         Lv& F = lv[l];
@@ -526,6 +527,7 @@ struct UlvImpl : UlvBase {
           comb_bwd<T, S>(H, F.vecM, F.d_toffB.p, Ustore(F), F.X.p, lx, F.Z.p, lz, tbuf.p, tlen, out, lo, nr, s);
         }
       }
+      whole.end();
       convert2d(x + c0 * ldx, ldx, v, n, n, nr, s);
     }
   }

@@ -116,14 +116,15 @@ class Resident:
     def solve(self, B: np.ndarray, tol: float, restart: int = 30, max_cycles: int = 20):
         """X for the columns of B (n x k, k <= max_rhs): FP64 GMRES (CGS2)
         with FP32 inner operator applications against the FP32 factors, in
-        the problem's resident right-hand-side buffers (no allocation)."""
+        the problem's resident right-hand-side buffers (no allocation); the
+        inner steps after a call's first replay one captured HIP graph."""
         n, k = B.shape
         assert n == self.spec.n and 1 <= k <= self.max_rhs
         B = np.asfortranarray(B, dtype=np.float64)
         G, H = N.gpu(), N.h2()
         N.check_gpu(G.mxg_memcpy_h2d(self.ctx.handle, self.dB.ptr, B.ctypes.data, B.nbytes))
         N.check_gpu(G.mxg_memset(self.ctx.handle, self.dX.ptr, 0, B.nbytes))
-        o = N.MxgGmresOpts(tol, restart, max_cycles, N.GMRES_CGS2, N.MXG_FP32, None)
+        o = N.MxgGmresOpts(tol, restart, max_cycles, N.GMRES_CGS2, N.MXG_FP32, None, 1)  # inner steps as a graph
         resid = np.zeros((k, max_cycles + 1))
         it = np.zeros(k, np.int64)
         inn = np.zeros(k, np.int64)

@@ -158,6 +158,25 @@ class H2:
     def matvec(self, x: DeviceArray, y: DeviceArray, precision: int = N.MXG_FP64) -> None:
         N.check_h2(N.h2().mxc_solver_matvec(self.handle, x.ptr, y.ptr, precision))
 
+    def autotune(self, precision: int = N.MXG_FP64, nrhs: int = 8, reps: int = 3):
+        """Times the matvec's candidate launch shapes for nrhs right-hand sides
+        and keeps the fastest (mxg_h2_autotune); returns (ms per product,
+        tuning dict)."""
+        ms = C.c_double()
+        dev = N.h2().mxc_solver_device(self.handle)
+        N.check_gpu(N.gpu().mxg_h2_autotune(dev, precision, nrhs, reps, C.byref(ms)))
+        return ms.value, self.tuning(precision)
+
+    def tuning(self, precision: int = N.MXG_FP64) -> dict:
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        dev = N.h2().mxc_solver_device(self.handle)
+        N.check_gpu(N.gpu().mxg_h2_get_tuning(dev, precision, C.byref(a), C.byref(b), C.byref(c)))
+        return {"near_symmetric": a.value, "far_symmetric": b.value, "split_rows": c.value}
+
+    def set_tuning(self, precision: int, near_symmetric: int, far_symmetric: int, split_rows: int) -> None:
+        dev = N.h2().mxc_solver_device(self.handle)
+        N.check_gpu(N.gpu().mxg_h2_set_tuning(dev, precision, near_symmetric, far_symmetric, split_rows))
+
     def factorize(self, precision: int = N.MXG_FP64) -> np.ndarray:
         info = np.zeros(self.levels + 1, np.int32)
         N.check_h2(N.h2().mxc_solver_factorize(self.handle, precision, info.ctypes.data))
@@ -198,15 +217,18 @@ class H2:
         return resid[: it.value + 1], it.value, inner.value
 
     def gmres_batch(self, M: "H2", B: DeviceArray, X: DeviceArray, tol: float, restart: int = 10, max_cycles: int = 50,
-                    mode: int = N.GMRES_REFERENCE, inner_precision: int = N.MXG_FP64, inner: "H2" = None):
+                    mode: int = N.GMRES_REFERENCE, inner_precision: int = N.MXG_FP64, inner: "H2" = None,
+                    graph: bool = False):
         """GMRES for the columns of B (n x nrhs) in lockstep, solutions into X
         (initial guesses on entry). Returns the residual histories (nrhs x
-        (max_cycles + 1)), cycles and inner iterations per right-hand side."""
+        (max_cycles + 1)), cycles and inner iterations per right-hand side.
+        graph: replay the inner steps' launches as one HIP graph (CGS2 with
+        FP32 inner iterations)."""
         n = self.problem.n
         nrhs = B.shape[1] if len(B.shape) > 1 else 1
         assert X.shape == B.shape and B.shape[0] == n
         inner_dev = N.h2().mxc_solver_device(inner.handle) if inner is not None else None
-        o = N.MxgGmresOpts(tol, restart, max_cycles, mode, inner_precision, inner_dev)
+        o = N.MxgGmresOpts(tol, restart, max_cycles, mode, inner_precision, inner_dev, int(graph))
         resid = np.zeros((nrhs, max_cycles + 1))
         it = np.zeros(nrhs, np.int64)
         inn = np.zeros(nrhs, np.int64)

@@ -88,3 +88,19 @@ def test_leveled_rank_grows_per_level(ctx):
     assert grew
     M.close()
     prob.close()
+
+
+@pytest.mark.timeout(600)
+def test_serial_sweep_log_and_csv(tmp_path):
+    from mxc_amd.sweep import serial
+
+    log, csv = serial(str(tmp_path / "log"), sizes=(2048, 4096))
+    text = open(log).read()
+    assert text.count("Construct Err") == 2 and text.count("GMRES Residual") == 2
+    rows = [r.split(",") for r in open(csv).read().splitlines()]
+    assert [r[1] for r in rows] == ["2048", "4096"]
+    for r in rows:
+        assert r[2] == "2" and r[3] == "256" and r[4] == "100" and float(r[5]) == 1e-10
+        iters = int(float(r[22]))
+        assert len(r) == 25 + iters + 1
+        assert float(r[21]) < 1e-10
