@@ -72,6 +72,16 @@ int mxg_memset(mxg_ctx_t ctx, void* d_dst, int value, size_t bytes);
 int mxg_gemm(mxg_ctx_t ctx, int precision, int ta, int tb, long long m, long long n, long long k, double alpha,
              const void* d_A, long long lda, const void* d_B, long long ldb, double beta, void* d_C, long long ldc);
 
+/* The solve phase's block product on one block, for checks of its own:
+ * Y[:, q] = B X[:, q] (q < nrhs, 1..8 right-hand sides) for an n x m column-major
+ * B (leading dimension ldb), with the lane layout the matvec and the
+ * substitution use for a block of this size (one wave, or a workgroup).
+ * storage MXG_FP64: B, X, Y double; MXG_FP32: float; MXG_BF16: B is bfloat16
+ * (uint16 holding the upper half of the FP32 pattern), X and Y float, FP32
+ * arithmetic. No counterpart in the reference (cuSPARSE SpMV, device_csr_matrix.cu). */
+int mxg_block_gemv(mxg_ctx_t ctx, int storage, long long n, long long m, const void* d_B, long long ldb,
+                   const void* d_X, long long ldx, void* d_Y, long long ldy, long long nrhs);
+
 /* ---- kernel blocks: replaces gen_matrix (kernel.cpp:71-85) on the device.
  * A[i + j*lda] = K(|bi_i - bj_j|), column major, points 3 doubles each. ---- */
 int mxg_gen_block(mxg_ctx_t ctx, mxg_kernel_t k, long long m, long long n, const double* d_bi,

@@ -94,6 +94,7 @@ def gpu() -> C.CDLL:
             "mxg_dense_matvec": [P, MxgKernel, P, c_ll, P, P],
             "mxg_dense_matvec_batch": [P, MxgKernel, P, c_ll, P, c_ll, P, c_ll, c_ll],
             "mxg_dense_matvec_batch_noalloc": [P, MxgKernel, P, c_ll, P, c_ll, P, c_ll, c_ll],
+            "mxg_block_gemv": [P, c_int, c_ll, c_ll, P, c_ll, P, c_ll, P, c_ll, c_ll],
             "mxg_h2_create": [P, MxgKernel, P, c_ll, c_ll, c_int, C.POINTER(P)],
             "mxg_h2_destroy": [P],
             "mxg_h2_define_level": [P, c_ll, c_ll, P, P, P, P, P, P],

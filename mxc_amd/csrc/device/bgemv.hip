@@ -340,6 +340,207 @@ __device__ inline void wave_gemv(const PairJob& t, int c0, T* __restrict__ vw) {
     }
 }
 
+// ---- workgroup jobs on the matrix cores (blocks stored in the compute type) ----
+typedef float mf4_t __attribute__((ext_vector_type(4)));
+typedef double md4_t __attribute__((ext_vector_type(4)));
+template <class T> struct MAcc;
+template <> struct MAcc<float> { typedef mf4_t type; };
+template <> struct MAcc<double> { typedef md4_t type; };
+__device__ inline mf4_t mfma16(float a, float b, mf4_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+__device__ inline md4_t mfma16(double a, double b, md4_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+// B[0:n, 0:m] v for NR right-hand sides on v_mfma_{f32_16x16x4f32,
+// f64_16x16x4f64}: out (16 rows x 16 right-hand sides) += B (16 rows x 4
+// columns) * v (4 columns x 16 right-hand sides, zero past NR). Wave w owns
+// rows [64 w, 64 w + 64) of each 256-row chunk; lane l loads the 16-byte
+// vectors of rows 16 V h + V (l % 16) + [0, V) of column c + l / 16 (V = 4
+// floats or 2 doubles, h < 4 / V), so a wave reads 64 rows of 4 columns per
+// step and element e of vector h is the A operand of row tile V h + e, whose
+// rows are 16 V h + V p + e, p = 0..15. The right-hand sides of the staged
+// columns sit in LDS as [c][16]; one read per lane gives the B operand of all
+// four tiles. No cross-wave sum: every row is one wave's chain of MFMAs over
+// the columns in order, so a result does not depend on NR.
+constexpr int MF_U = 4;  // column steps (4 columns each) whose loads are in flight
+template <class T, int NR>
+__device__ inline void wg_gemv_mfma(const T* __restrict__ B, long long ld, int n, int m, const T* __restrict__ vb,
+                                    long long ldv, T* __restrict__ ob, long long ldo, T* xs) {
+  constexpr int V = 16 / (int)sizeof(T);
+  constexpr int H = 4 / V;  // vectors per lane and column step
+  constexpr int CW = lds_elems<T>() / 16;  // staged columns per chunk
+  typedef T VT __attribute__((ext_vector_type(V)));
+  typedef typename MAcc<T>::type AT;
+  const int lane = threadIdx.x & 63, wave = wave_id();
+  const int lc = lane >> 4, lr = lane & 15;
+  for (int r0 = 0; r0 < n; r0 += 256) {
+    const int rbase = r0 + 64 * wave;
+    AT acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t] = AT(0);
+    for (int cb = 0; cb < m; cb += CW) {
+      const int cn = min(CW, m - cb);
+      const int cn4 = (cn + 3) & ~3;
+      __syncthreads();  // the previous chunk's reads of xs are done
+      for (int c = threadIdx.x; c < cn4; c += 256) {
+        T row[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) row[k] = (k < NR && c < cn) ? vb[cb + c + (long long)k * ldv] : T(0);
+#pragma unroll
+        for (int k = 0; k < 16; k += V) {
+          VT v;
+#pragma unroll
+          for (int e = 0; e < V; e++) v[e] = row[k + e];
+          *reinterpret_cast<VT*>(xs + c * 16 + k) = v;
+        }
+      }
+      __syncthreads();
+      if (rbase < n) {  // wave-uniform
+        const T* Bc = B + (long long)cb * ld + rbase + V * lr;
+        int c = 0;
+        for (; c + 4 * (MF_U - 1) < cn4; c += 4 * MF_U) {
+          VT a[MF_U][H];
+#pragma unroll
+          for (int u = 0; u < MF_U; u++) {
+            const int col = c + 4 * u + lc;
+#pragma unroll
+            for (int h = 0; h < H; h++) {
+              const int row = rbase + 16 * V * h + V * lr;
+              a[u][h] = (col < cn && row < n) ? *reinterpret_cast<const VT*>(Bc + (long long)col * ld + 16 * V * h)
+                                              : VT(0);
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < MF_U; u++) {
+            const T bop = xs[(c + 4 * u + lc) * 16 + lr];
+#pragma unroll
+            for (int h = 0; h < H; h++)
+#pragma unroll
+              for (int e = 0; e < V; e++) acc[V * h + e] = mfma16(a[u][h][e], bop, acc[V * h + e]);
+          }
+        }
+        for (; c < cn4; c += 4) {
+          const int col = c + lc;
+          VT a[H];
+#pragma unroll
+          for (int h = 0; h < H; h++) {
+            const int row = rbase + 16 * V * h + V * lr;
+            a[h] = (col < cn && row < n) ? *reinterpret_cast<const VT*>(Bc + (long long)col * ld + 16 * V * h) : VT(0);
+          }
+          const T bop = xs[col * 16 + lr];
+#pragma unroll
+          for (int h = 0; h < H; h++)
+#pragma unroll
+            for (int e = 0; e < V; e++) acc[V * h + e] = mfma16(a[h][e], bop, acc[V * h + e]);
+        }
+      }
+    }
+    if (rbase < n) {
+      // entry q of tile V h + e in lane l: tile row p = 4 (l / 16) + q (FP32) or
+      // l / 16 + 4 q (FP64), right-hand side l % 16
+#pragma unroll
+      for (int h = 0; h < H; h++)
+#pragma unroll
+        for (int e = 0; e < V; e++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int p = sizeof(T) == 4 ? 4 * lc + q : lc + 4 * q;
+            const int row = rbase + 16 * V * h + V * p + e;
+            if (lr < NR && row < n) ob[row + (long long)lr * ldo] = acc[V * h + e][q];
+          }
+    }
+  }
+}
+
+// A wave-sized job on the matrix cores: the same operand layout as
+// wg_gemv_mfma, with the whole block (n <= 64 V rows) on one wave in row
+// groups of 64, and blocks stored as S converted to T on load (bfloat16 is
+// the upper half of the FP32 pattern, so the conversion is exact). The
+// right-hand sides of up to CW columns are staged in the wave's quarter of
+// LDS as [c][16].
+template <class T, class S, int NR>
+__device__ inline void wave_gemv_mfma(const PairJob& t, int c0, T* __restrict__ vw) {
+  constexpr int V = std::is_same<S, bf16_t>::value ? 4 : 16 / (int)sizeof(T);  // elements per vector load
+  constexpr int H = 4 / V;
+  constexpr int G = sizeof(T) == 4 ? 4 : 2;  // row groups of 64 rows: a wave job has n <= 64 V
+  constexpr int CW = lds_elems<T>() / 4 / 16;
+  typedef typename Raw<T, S, V>::type RT;
+  typedef typename MAcc<T>::type AT;
+  static_assert(H * V == 4, "a 16-lane group covers 16 V rows; H vectors make 64");
+  const S* B = static_cast<const S*>(t.B);
+  const T* vb = static_cast<const T*>(t.v) + (long long)c0 * t.ldv;
+  T* ob = static_cast<T*>(t.out) + (long long)c0 * t.ldo;
+  const int lane = threadIdx.x & 63, lc = lane >> 4, lr = lane & 15;
+  const int n = t.n, m = t.m;
+  const int ng = (n + 63) / 64;  // wave-uniform
+  AT acc[G][4];
+#pragma unroll
+  for (int g = 0; g < G; g++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[g][q] = AT(0);
+  for (int cb = 0; cb < m; cb += CW) {
+    const int cn = min(CW, m - cb);
+    const int cn4 = (cn + 3) & ~3;
+    wave_lds_sync();
+    for (int c = lane; c < cn4; c += 64)
+#pragma unroll
+      for (int k = 0; k < 16; k++) vw[c * 16 + k] = (k < NR && c < cn) ? vb[cb + c + (long long)k * t.ldv] : T(0);
+    wave_lds_sync();
+#pragma unroll 4
+    for (int c = 0; c < cn4; c += 4) {
+      const int col = c + lc;
+      const T bop = vw[col * 16 + lr];
+      const S* Bc = B + (long long)(cb + col) * t.ld + V * lr;
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        if (g < ng) {
+#pragma unroll
+          for (int h = 0; h < H; h++) {
+            const int row = 64 * g + 16 * V * h + V * lr;
+            const auto a = cvt<T, S, V>((col < cn && row < n) ? load_raw<T, S, V>(Bc + 64 * g + 16 * V * h) : RT(0));
+#pragma unroll
+            for (int e = 0; e < V; e++) acc[g][V * h + e] = mfma16(a[e], bop, acc[g][V * h + e]);
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < G; g++)
+    if (g < ng)
+#pragma unroll
+      for (int h = 0; h < H; h++)
+#pragma unroll
+        for (int e = 0; e < V; e++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int p = sizeof(T) == 4 ? 4 * lc + q : lc + 4 * q;
+            const int row = 64 * g + 16 * V * h + V * p + e;
+            if (lr < NR && row < n) ob[row + (long long)lr * t.ldo] = acc[g][V * h + e][q];
+          }
+}
+
+// The matrix-core jobs (mx = 1) of a list: right-hand sides c0 .. c0 + NR - 1;
+// workgroups below nbig take one workgroup-sized job each, the others four
+// wave-sized jobs. A kernel of its own, so that the vector-unit paths' register
+// needs do not lower this one's occupancy.
+template <class T, class S, int NR>
+__global__ void __launch_bounds__(256) k_pair_gemv_mx(const PairJob* __restrict__ jobs, long long nbig, long long count,
+                                                      int c0) {
+  __shared__ __align__(16) T vs[lds_elems<T>()];
+  if ((long long)blockIdx.x >= nbig) {
+    const int wave = wave_id();
+    const long long j = nbig + ((long long)blockIdx.x - nbig) * 4 + wave;
+    if (j >= count) return;
+    wave_gemv_mfma<T, S, NR>(jobs[j], c0, vs + wave * (lds_elems<T>() / 4));
+    return;
+  }
+  if constexpr (std::is_same<T, S>::value) {
+    const PairJob t = jobs[blockIdx.x];
+    const T* vb = static_cast<const T*>(t.v) + (long long)c0 * t.ldv;
+    T* ob = static_cast<T*>(t.out) + (long long)c0 * t.ldo;
+    wg_gemv_mfma<T, NR>(static_cast<const T*>(t.B), t.ld, t.n, t.m, vb, t.ldv, ob, t.ldo, vs);
+  }
+}
+
 // Right-hand sides c0 .. c0 + NR - 1 of every job. Workgroups below nbig take
 // one workgroup-sized job each; the others take four wave-sized jobs.
 template <class T, class S, int NR>
@@ -547,7 +748,10 @@ void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& t
     const int vec = ((reinterpret_cast<uintptr_t>(b) % 16) == 0 && t.ld % W == 0 && t.n % W == 0) ? 1 : 0;
     const long long wave_bytes = std::is_same<S, bf16_t>::value ? 2 * WAVE_JOB_BYTES : WAVE_JOB_BYTES;
     const int wave = (t.n <= 64 * (vec ? W : 1) && (long long)t.n * t.m * (long long)sizeof(S) <= wave_bytes) ? 1 : 0;
-    jobs.push_back(PairJob{b, v + t.v, out + t.out, ldv, ldo, t.n, t.m, t.ld, vec, wave});
+    // the matrix cores take aligned blocks: wave jobs of any storage, and
+    // workgroup jobs stored in the compute type
+    const int mx = (vec && (wave || std::is_same<T, S>::value)) ? 1 : 0;
+    jobs.push_back(PairJob{b, v + t.v, out + t.out, ldv, ldo, t.n, t.m, t.ld, vec, wave, mx});
   }
 }
 template void pair_jobs_append<float, float>(std::vector<PairJob>&, const std::vector<PairTask>&, const float*, long long,
@@ -562,11 +766,15 @@ void pair_list_upload(PairList& pl, const std::vector<PairJob>& jobs, hipStream_
   MXG_REQUIRE(jobs.size() < (1ull << 31), "pair list too long");
   std::vector<PairJob> sorted;
   sorted.reserve(jobs.size());
-  for (const PairJob& j : jobs)
-    if (!j.wave) sorted.push_back(j);
-  pl.nbig = (long long)sorted.size();
-  for (const PairJob& j : jobs)
-    if (j.wave) sorted.push_back(j);
+  for (int mx : {1, 0}) {
+    for (const PairJob& j : jobs)
+      if (j.mx == mx && !j.wave) sorted.push_back(j);
+    if (mx) pl.nbig_mx = (long long)sorted.size();
+    else pl.nbig = (long long)sorted.size() - pl.count_mx;
+    for (const PairJob& j : jobs)
+      if (j.mx == mx && j.wave) sorted.push_back(j);
+    if (mx) pl.count_mx = (long long)sorted.size();
+  }
   pl.count = (long long)sorted.size();
   if (pl.count) pl.jobs.upload(sorted, s);
 }
@@ -575,18 +783,36 @@ template <class T, class S>
 void pair_gemv(const PairList& pl, int nrhs, hipStream_t s) {
   if (pl.count == 0) return;
   MXG_REQUIRE(nrhs >= 1 && nrhs <= MAX_RHS, "right-hand sides out of range");
-  const long long groups = pl.nbig + (pl.count - pl.nbig + 3) / 4;
-  MXG_REQUIRE(groups < (1ll << 31), "pair list too long");
+  // matrix-core jobs [0, count_mx), vector-unit jobs [count_mx, count)
+  const PairJob* jm = pl.jobs.p;
+  const PairJob* jv = pl.jobs.p + pl.count_mx;
+  const long long nv = pl.count - pl.count_mx;
+  const long long gm = pl.nbig_mx + (pl.count_mx - pl.nbig_mx + 3) / 4;
+  const long long gv = pl.nbig + (nv - pl.nbig + 3) / 4;
+  MXG_REQUIRE(gm < (1ll << 31) && gv < (1ll << 31), "pair list too long");
   for (int c0 = 0; c0 < nrhs;) {
     const int nr = batch_width(nrhs - c0);
-    const dim3 g((unsigned)groups), b(256);
-    switch (nr) {
-      case 8: hipLaunchKernelGGL((k_pair_gemv<T, S, 8>), g, b, 0, s, pl.jobs.p, pl.nbig, pl.count, c0); break;
-      case 4: hipLaunchKernelGGL((k_pair_gemv<T, S, 4>), g, b, 0, s, pl.jobs.p, pl.nbig, pl.count, c0); break;
-      case 2: hipLaunchKernelGGL((k_pair_gemv<T, S, 2>), g, b, 0, s, pl.jobs.p, pl.nbig, pl.count, c0); break;
-      default: hipLaunchKernelGGL((k_pair_gemv<T, S, 1>), g, b, 0, s, pl.jobs.p, pl.nbig, pl.count, c0); break;
+    const dim3 b(256);
+    if (gm > 0) {
+      const dim3 g((unsigned)gm);
+      switch (nr) {
+        case 8: hipLaunchKernelGGL((k_pair_gemv_mx<T, S, 8>), g, b, 0, s, jm, pl.nbig_mx, pl.count_mx, c0); break;
+        case 4: hipLaunchKernelGGL((k_pair_gemv_mx<T, S, 4>), g, b, 0, s, jm, pl.nbig_mx, pl.count_mx, c0); break;
+        case 2: hipLaunchKernelGGL((k_pair_gemv_mx<T, S, 2>), g, b, 0, s, jm, pl.nbig_mx, pl.count_mx, c0); break;
+        default: hipLaunchKernelGGL((k_pair_gemv_mx<T, S, 1>), g, b, 0, s, jm, pl.nbig_mx, pl.count_mx, c0); break;
+      }
+      MXG_HIP(hipGetLastError());
     }
-    MXG_HIP(hipGetLastError());
+    if (gv > 0) {
+      const dim3 g((unsigned)gv);
+      switch (nr) {
+        case 8: hipLaunchKernelGGL((k_pair_gemv<T, S, 8>), g, b, 0, s, jv, pl.nbig, nv, c0); break;
+        case 4: hipLaunchKernelGGL((k_pair_gemv<T, S, 4>), g, b, 0, s, jv, pl.nbig, nv, c0); break;
+        case 2: hipLaunchKernelGGL((k_pair_gemv<T, S, 2>), g, b, 0, s, jv, pl.nbig, nv, c0); break;
+        default: hipLaunchKernelGGL((k_pair_gemv<T, S, 1>), g, b, 0, s, jv, pl.nbig, nv, c0); break;
+      }
+      MXG_HIP(hipGetLastError());
+    }
     c0 += nr;
   }
 }
@@ -722,3 +948,40 @@ void to_bf16(bf16_t* dst, const float* src, size_t n, hipStream_t s) {
 }
 
 }  // namespace mxg
+
+using namespace mxg;
+
+namespace {
+template <class T, class S>
+void one_block(mxg_ctx_t ctx, long long n, long long m, const void* B, long long ldb, const void* X, long long ldx, void* Y,
+               long long ldy, int nrhs) {
+  std::vector<PairJob> jobs;
+  const std::vector<PairTask> t{PairTask{0, 0, 0, (int)n, (int)m, (int)ldb, 0}};
+  pair_jobs_append<T, S>(jobs, t, static_cast<const S*>(B), (m - 1) * ldb + n, static_cast<const T*>(X), m, ldx,
+                         static_cast<T*>(Y), n, ldy);
+  PairList pl;
+  pair_list_upload(pl, jobs, ctx->stream);
+  pair_gemv<T, S>(pl, nrhs, ctx->stream);
+  MXG_HIP(hipStreamSynchronize(ctx->stream));
+}
+}  // namespace
+
+extern "C" int mxg_block_gemv(mxg_ctx_t ctx, int storage, long long n, long long m, const void* d_B, long long ldb,
+                              const void* d_X, long long ldx, void* d_Y, long long ldy, long long nrhs) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx, "ctx is null");
+    MXG_REQUIRE(n >= 1 && m >= 1 && n < (1 << 30) && m < (1 << 30), "block size out of range");
+    MXG_REQUIRE(ldb >= n && ldx >= m && ldy >= n && ldb < (1 << 30), "bad leading dimensions");
+    MXG_REQUIRE(nrhs >= 1 && nrhs <= MAX_RHS, "nrhs must be 1..8");
+    MXG_REQUIRE(d_B && d_X && d_Y, "null pointer");
+    MXG_HIP(hipSetDevice(ctx->device));
+    if (storage == MXG_FP64)
+      one_block<double, double>(ctx, n, m, d_B, ldb, d_X, ldx, d_Y, ldy, (int)nrhs);
+    else if (storage == MXG_FP32)
+      one_block<float, float>(ctx, n, m, d_B, ldb, d_X, ldx, d_Y, ldy, (int)nrhs);
+    else if (storage == MXG_BF16)
+      one_block<float, bf16_t>(ctx, n, m, d_B, ldb, d_X, ldx, d_Y, ldy, (int)nrhs);
+    else
+      throw Error(MXG_ERR_ARG, "unknown storage");
+  });
+}

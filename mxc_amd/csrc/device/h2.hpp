@@ -63,6 +63,7 @@ struct PairJob {
   int n, m, ld;
   int vec;   // 1: B 16-byte aligned with ld and n multiples of the vector width
   int wave;  // 1: small enough for one wave (handled four to a workgroup)
+  int mx;    // 1: runs on the matrix cores (k_pair_gemv_mx), else on the vector units
 };
 // right-hand sides one launch of the solve-phase kernels carries at most
 constexpr int MAX_RHS = 8;
@@ -94,11 +95,14 @@ struct SymList {
   DevBuf<SymJob> jobs;
   long long count = 0;
 };
-// Jobs of one launch: the workgroup-sized ones first, then the wave-sized.
+// Jobs of one list: those for the matrix cores first (workgroup-sized, then
+// wave-sized), then those for the vector units (workgroup-sized, then
+// wave-sized); each group is one launch.
 struct PairList {
   DevBuf<PairJob> jobs;
-  long long count = 0;
-  long long nbig = 0;
+  long long count = 0;                   // all jobs
+  long long count_mx = 0, nbig_mx = 0;   // jobs [0, count_mx), the first nbig_mx workgroup-sized
+  long long nbig = 0;                    // workgroup-sized jobs among [count_mx, count)
 };
 
 struct Level {

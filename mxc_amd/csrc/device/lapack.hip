@@ -316,6 +316,176 @@ __global__ void __launch_bounds__(256) k_lu_inverse(const LuTask<T>* __restrict_
   if (threadIdx.x == 0) *t.info = (singular || logdet <= log(DBL_MIN)) ? 1 : 0;
 }
 
+// The same LU with partial pivoting and explicit inverse, with the matrix in
+// registers: 512 threads form a 16 x 32 grid and thread (ti, tj) = (t % 16,
+// t / 16) owns the TR x TC tile of rows ti TR .. and columns tj TC .. (n <=
+// 16 TR = 32 TC); at two waves per SIMD a thread may hold 256 registers, so
+// a 192 x 192 FP64 block (TR 12, TC 6: 144 registers of tile) stays on
+// chip. Every step of the LU (pivot search, row swap, scaling of column k,
+// rank-1 update) and of the two triangular sweeps of the inverse touches LDS
+// only for the pivot column and the row it broadcasts, double-buffered by
+// the parity of the step, so each step costs one or two workgroup barriers;
+// W goes to global memory once, between the LU and the sweeps, which read
+// its columns. The arithmetic per element and its order are those of
+// k_lu_inverse. Tile indices into the register arrays are compile-time; no
+// register is written under a condition on a runtime row index (that would
+// turn the tile into an indexed array in scratch memory), so row swaps are
+// logical (pos) and the U sweep divides a row when it is written out.
+template <class T, int TR, int TC>
+__global__ void __launch_bounds__(512) k_lu_inverse_reg(const LuTask<T>* __restrict__ tasks) {
+  constexpr int NP = 16 * TR;
+  static_assert(16 * TR == 32 * TC, "square coverage");
+  __shared__ T colb[2][NP];   // column k
+  __shared__ T rowp[2][NP];   // the pivot row (the new row k)
+  __shared__ int pos[2][NP];  // logical row of each stored row, before / after step k's swap
+  __shared__ int perm[NP];
+  __shared__ T diag[NP];
+  const LuTask<T> task = tasks[blockIdx.x];
+  const int n = task.n;
+  const int tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
+  const int r0 = ti * TR, c0 = tj * TC;
+  const int lane = tid & 63;
+  T a[TR][TC];
+#pragma unroll
+  for (int i = 0; i < TR; i++)
+#pragma unroll
+    for (int j = 0; j < TC; j++)
+      a[i][j] = (r0 + i < n && c0 + j < n) ? task.A[(r0 + i) + (long long)(c0 + j) * task.lda] : T(0);
+  double logdet = 0.;
+  int singular = 0;
+  for (int i = tid; i < NP; i += 512) pos[0][i] = i;
+  // Rows never move: a swap of logical rows k and p exchanges their logical
+  // positions (pos) instead, and a stored row takes part in step k's update
+  // while its logical position is above k.
+  for (int k = 0; k < n; k++) {
+    const int b = k & 1;
+    if (tj == k / TC)
+#pragma unroll
+      for (int i = 0; i < TR; i++)
+#pragma unroll
+        for (int j = 0; j < TC; j++)
+          if (c0 + j == k) colb[b][r0 + i] = a[i][j];
+    __syncthreads();
+    // pivot: the largest |a| among rows at logical position >= k, ties to the
+    // smallest logical position (every wave finds it)
+    double bv = -1.;
+    int bl = n, br = n;
+    for (int r = lane; r < n; r += 64) {
+      const int l = pos[b][r];
+      if (l < k) continue;
+      const double v = fabs((double)colb[b][r]);
+      if (v > bv || (v == bv && l < bl)) { bv = v; bl = l; br = r; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(bv, o, 64);
+      const int ol = __shfl_xor(bl, o, 64), orr = __shfl_xor(br, o, 64);
+      if (ov > bv || (ov == bv && ol < bl)) { bv = ov; bl = ol; br = orr; }
+    }
+    const int pr = br;  // the stored row that becomes logical row k
+    // the stored row now at logical k moves to the pivot's position bl
+    for (int r = tid; r < NP; r += 512) {
+      const int l = pos[b][r];
+      pos[b ^ 1][r] = (r == pr) ? k : (l == k ? bl : l);
+    }
+#pragma unroll
+    for (int i = 0; i < TR; i++)
+#pragma unroll
+      for (int j = 0; j < TC; j++)
+        if (r0 + i == pr) rowp[b][c0 + j] = a[i][j];
+    __syncthreads();
+    const T d = rowp[b][k];
+    const bool upd = d != T(0);
+    if (!upd) singular = 1;
+    else logdet += log(fabs((double)d));
+    const T inv = upd ? T(1) / d : T(0);
+#pragma unroll
+    for (int i = 0; i < TR; i++) {
+      const int r = r0 + i;
+      const bool below = upd && r < n && pos[b ^ 1][r] > k;  // rows the step updates
+      const T l = colb[b][r] * inv;
+#pragma unroll
+      for (int j = 0; j < TC; j++) {
+        const int c = c0 + j;
+        T v = a[i][j];
+        if (below) {
+          if (c == k) v = l;
+          else if (c > k && c < n) v -= l * rowp[b][c];
+        }
+        a[i][j] = v;
+      }
+    }
+  }
+  // W in logical row order to the work buffer (the sweeps read its columns),
+  // its diagonal and the row permutation P to LDS
+  const int* fpos = pos[n & 1];
+  for (int r = tid; r < n; r += 512) perm[fpos[r]] = r;
+#pragma unroll
+  for (int i = 0; i < TR; i++)
+#pragma unroll
+    for (int j = 0; j < TC; j++)
+      if (r0 + i < n && c0 + j < n) {
+        const int l = fpos[r0 + i];
+        task.work[l + (long long)(c0 + j) * n] = a[i][j];
+        if (l == c0 + j) diag[l] = a[i][j];
+      }
+  __syncthreads();
+  // X = P I, then L^{-1}, then U^{-1}; x reuses a's registers
+  auto& x = a;
+#pragma unroll
+  for (int i = 0; i < TR; i++)
+#pragma unroll
+    for (int j = 0; j < TC; j++) x[i][j] = (r0 + i < n && c0 + j == perm[r0 + i]) ? T(1) : T(0);
+  const T* W = task.work;
+  for (int k = 0; k < n - 1; k++) {
+    const int b = k & 1;
+#pragma unroll
+    for (int i = 0; i < TR; i++)
+#pragma unroll
+      for (int j = 0; j < TC; j++)
+        if (r0 + i == k) colb[b][c0 + j] = x[i][j];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TR; i++) {
+      const int r = r0 + i;
+      if (r > k && r < n) {
+        const T l = W[r + (long long)k * n];
+#pragma unroll
+        for (int j = 0; j < TC; j++) x[i][j] -= l * colb[b][c0 + j];
+      }
+    }
+  }
+  // U^{-1}: step k divides row k by U_kk and subtracts it from the rows
+  // above. Row k no longer changes after step k, so its registers keep the
+  // undivided values and are divided once when written out; the divided row
+  // goes to the other threads through LDS.
+  for (int k = n - 1; k >= 0; k--) {
+    const int b = k & 1;
+    const T dk = diag[k];
+#pragma unroll
+    for (int i = 0; i < TR; i++)
+#pragma unroll
+      for (int j = 0; j < TC; j++)
+        if (r0 + i == k) rowp[b][c0 + j] = x[i][j] / dk;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TR; i++) {
+      const int r = r0 + i;
+      if (r < k) {
+        const T u = W[r + (long long)k * n];
+#pragma unroll
+        for (int j = 0; j < TC; j++) x[i][j] -= u * rowp[b][c0 + j];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TR; i++)
+#pragma unroll
+    for (int j = 0; j < TC; j++)
+      if (r0 + i < n && c0 + j < n) task.inv[(r0 + i) + (long long)(c0 + j) * n] = x[i][j] / diag[r0 + i];
+  if (tid == 0) *task.info = (singular || logdet <= log(DBL_MIN)) ? 1 : 0;
+}
+
 template <class Task, class Kern>
 void launch_tasks(const std::vector<Task>& tasks, Kern kern, hipStream_t s) {
   if (tasks.empty()) return;
@@ -338,10 +508,42 @@ void cpqr_batched(const std::vector<CpqrTask>& tasks, hipStream_t s) {
   launch_tasks(tasks, k_cpqr, s);
 }
 
+// Tile shapes of the register LU (512 threads, n <= 16 TR): FP64 up to
+// 192 rows (144 registers of tile per thread), FP32 up to 256.
+template <class T>
+constexpr int lu_max_rows() {
+  return sizeof(T) == 8 ? 192 : 256;
+}
+
+template <class Task, class Kern>
+void launch_tasks_512(const std::vector<Task>& tasks, Kern kern, hipStream_t s) {
+  if (tasks.empty()) return;
+  MXG_REQUIRE(tasks.size() < (1u << 31), "too many tasks");
+  DevBuf<Task> dt;
+  dt.upload(tasks, s);
+  hipLaunchKernelGGL(kern, dim3((unsigned)tasks.size()), dim3(512), 0, s, dt.p);
+  MXG_HIP(hipGetLastError());
+  MXG_HIP(hipStreamSynchronize(s));
+}
+
 template <class T>
 void lu_inverse_batched(const std::vector<LuTask<T>>& tasks, hipStream_t s) {
   for (const LuTask<T>& t : tasks) MXG_REQUIRE(t.n >= 0 && t.n <= 1024, "LU block larger than 1024");
-  launch_tasks(tasks, k_lu_inverse<T>, s);
+  // by size: the register kernel for each tile shape in use, the
+  // global-memory kernel above lu_max_rows
+  std::vector<LuTask<T>> by[4], big;
+  for (const LuTask<T>& t : tasks) {
+    if (t.n <= 64) by[0].push_back(t);
+    else if (t.n <= 128) by[1].push_back(t);
+    else if (t.n <= 192 && t.n <= lu_max_rows<T>()) by[2].push_back(t);
+    else if (t.n <= lu_max_rows<T>()) by[3].push_back(t);
+    else big.push_back(t);
+  }
+  launch_tasks_512(by[0], k_lu_inverse_reg<T, 4, 2>, s);
+  launch_tasks_512(by[1], k_lu_inverse_reg<T, 8, 4>, s);
+  launch_tasks_512(by[2], k_lu_inverse_reg<T, 12, 6>, s);
+  if constexpr (lu_max_rows<T>() >= 256) launch_tasks_512(by[3], k_lu_inverse_reg<T, 16, 8>, s);
+  launch_tasks(big, k_lu_inverse<T>, s);
 }
 
 template void lu_inverse_batched<double>(const std::vector<LuTask<double>>&, hipStream_t);

@@ -95,8 +95,9 @@ def test_serial_sweep_log_and_csv(tmp_path):
     from mxc_amd.sweep import serial
 
     log, csv = serial(str(tmp_path / "log"), sizes=(2048, 4096))
-    text = open(log).read()
-    assert text.count("Construct Err") == 2 and text.count("GMRES Residual") == 2
+    lines = open(log).read().splitlines()
+    assert sum(ln.startswith("Construct Err:") for ln in lines) == 2
+    assert sum(ln.startswith("GMRES Residual:") for ln in lines) == 2
     rows = [r.split(",") for r in open(csv).read().splitlines()]
     assert [r[1] for r in rows] == ["2048", "4096"]
     for r in rows:
