@@ -67,15 +67,15 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q))
 
 
-def measured_traffic(nr):
+def measured_traffic(nr, kernel="k_pair_gemv<float, float, 8>"):
     """HBM bytes per launch of the FP32 near-product launch with nr right-hand
     sides, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes (None when
-    absent or measured for another batch width)."""
+    absent, or measured for another kernel or batch width)."""
     path = os.path.join(ROOT, "profiles", "pair_gemv_matvec_fp32_traffic.md")
     try:
         text = open(path).read()
         rec = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
-        same = int(rec["rhs"]) == nr and rec.get("launch") == "leaf near products"
+        same = int(rec["rhs"]) == nr and rec.get("launch") == "leaf near products" and rec.get("kernel") == kernel
         return float(rec["traffic_bytes"]) if same else None
     except (OSError, IndexError, KeyError, ValueError):
         return None
@@ -266,6 +266,7 @@ def main():
                           "frac_of_fp32_mfma_peak": round(fflops / mixed["factor_s"] / 1e12 / 157.3, 3)},
         },
         "kernels_mixed": mixed["kernels"],
+        "kernels_fp64": fp64["kernels"],
         "matvec_autotune": {k: {"ms": round(v[0], 4), **v[1]} for k, v in tuning.items()},
         "operator_levels": sizes,
         "roofline": {

@@ -748,9 +748,11 @@ void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& t
     const int vec = ((reinterpret_cast<uintptr_t>(b) % 16) == 0 && t.ld % W == 0 && t.n % W == 0) ? 1 : 0;
     const long long wave_bytes = std::is_same<S, bf16_t>::value ? 2 * WAVE_JOB_BYTES : WAVE_JOB_BYTES;
     const int wave = (t.n <= 64 * (vec ? W : 1) && (long long)t.n * t.m * (long long)sizeof(S) <= wave_bytes) ? 1 : 0;
-    // the matrix cores take aligned blocks: wave jobs of any storage, and
-    // workgroup jobs stored in the compute type
-    const int mx = (vec && (wave || std::is_same<T, S>::value)) ? 1 : 0;
+    // the matrix cores take the aligned FP64 blocks; for FP32 and bfloat16
+    // blocks the vector-unit products measured faster (N = 2^20 sphere:
+    // near launch 1.89 against 2.22 ms, leaf substitution 0.82 against
+    // 1.76 ms), for FP64 the matrix cores (near launch 4.02 against 4.57 ms)
+    const int mx = (vec && sizeof(T) == 8 && std::is_same<T, S>::value) ? 1 : 0;
     jobs.push_back(PairJob{b, v + t.v, out + t.out, ldv, ldo, t.n, t.m, t.ld, vec, wave, mx});
   }
 }
