@@ -67,6 +67,16 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a), q))
 
 
+def profile_class(N, ctx, cls):
+    """Launches, summed device time and algorithmic work of one kernel class
+    since the last profile reset."""
+    import ctypes as C
+    cnt, ms, work = C.c_longlong(), C.c_double(), C.c_double()
+    N.check_gpu(N.gpu().mxg_profile_read(ctx.handle, cls, C.byref(cnt), C.byref(ms)))
+    N.check_gpu(N.gpu().mxg_profile_work(ctx.handle, cls, C.byref(work)))
+    return {"launches": cnt.value, "ms": ms.value, "work": work.value}
+
+
 def measured_traffic(nr, kernel="k_pair_gemv<float, float, 8>"):
     """HBM bytes per launch of the FP32 near-product launch with nr right-hand
     sides, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes (None when
@@ -112,9 +122,15 @@ def main():
     t0 = time.time()
     pts = generate_points("sphere", n, 2)
     prob = Problem(ctx, pts, leaf)
+    # the far-field sketches of the operator's construction are timed with
+    # HIP events (one pair per level) and their flops counted
+    N.check_gpu(N.gpu().mxg_profile_reset(ctx.handle))
+    N.check_gpu(N.gpu().mxg_profile_enable(ctx.handle, 1))
     A = H2.operator(prob, args.op_eps, args.sampling, kernel, diag, alpha, theta)
     t_op = time.time() - t0
-    log(f"operator built in {t_op:.1f} s")
+    sketch = profile_class(N, ctx, N.MXG_PROF_SKETCH)
+    N.check_gpu(N.gpu().mxg_profile_enable(ctx.handle, 0))
+    log(f"operator built in {t_op:.1f} s; sketch {sketch}")
     t0 = time.time()
     M = H2.preconditioner(prob, 64, args.sampling, kernel, diag, alpha, theta, eps=1e-6)
     t_pc = time.time() - t0
@@ -264,6 +280,14 @@ def main():
                      "frac_of_fp64_mfma_peak": round(fflops / fp64["factor_s"] / 1e12 / 78.6, 3)},
             "fp32_bf16": {"s": round(mixed["factor_s"], 4), "tflops": round(fflops / mixed["factor_s"] / 1e12, 2),
                           "frac_of_fp32_mfma_peak": round(fflops / mixed["factor_s"] / 1e12 / 157.3, 3)},
+        },
+        "construction": {
+            "operator_s": round(t_op, 2),
+            "sketch": {"kernel": "k_sketch (FP64 MFMA) + k_sum_slices", "launches": sketch["launches"],
+                       "s": round(sketch["ms"] * 1e-3, 3), "algorithmic_flops": sketch["work"],
+                       "tflops": round(sketch["work"] / (sketch["ms"] * 1e-3) / 1e12, 2) if sketch["ms"] else None,
+                       "frac_of_fp64_mfma_peak": round(sketch["work"] / (sketch["ms"] * 1e-3) / 1e12 / 78.6, 3)
+                       if sketch["ms"] else None},
         },
         "kernels_mixed": mixed["kernels"],
         "kernels_fp64": fp64["kernels"],

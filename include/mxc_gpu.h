@@ -78,9 +78,13 @@ int mxg_gemm(mxg_ctx_t ctx, int precision, int ta, int tb, long long m, long lon
  * substitution use for a block of this size (one wave, or a workgroup).
  * storage MXG_FP64: B, X, Y double; MXG_FP32: float; MXG_BF16: B is bfloat16
  * (uint16 holding the upper half of the FP32 pattern), X and Y float, FP32
- * arithmetic. No counterpart in the reference (cuSPARSE SpMV, device_csr_matrix.cu). */
+ * arithmetic. matrix_cores != 0: an FP64 or FP32 block whose columns admit
+ * 16-byte loads runs on the matrix cores (v_mfma_f64_16x16x4f64 /
+ * v_mfma_f32_16x16x4f32), as the matvec's near and coupling products do when
+ * its tuning selects them; otherwise on the vector units. No counterpart in
+ * the reference (cuSPARSE SpMV, device_csr_matrix.cu). */
 int mxg_block_gemv(mxg_ctx_t ctx, int storage, long long n, long long m, const void* d_B, long long ldb,
-                   const void* d_X, long long ldx, void* d_Y, long long ldy, long long nrhs);
+                   const void* d_X, long long ldx, void* d_Y, long long ldy, long long nrhs, int matrix_cores);
 
 /* ---- kernel blocks: replaces gen_matrix (kernel.cpp:71-85) on the device.
  * A[i + j*lda] = K(|bi_i - bj_j|), column major, points 3 doubles each. ---- */
@@ -170,14 +174,18 @@ int mxg_h2_upload(mxg_h2_t h, long long level, int what, const double* src, long
  * sides, reps products each after a warm-up, and keeps the fastest for later
  * matvecs. Candidates: the read-once symmetric products or the general ones
  * for the leaf near blocks (matrices built from a kernel) and for the
- * couplings (interpolative bases), and 16-, 32- or 64-row chunks for the
- * upward products of levels with fewer than 1024 blocks. Every shape gives
- * the same matvec up to rounding. best_ms: time per product of the kept shape. */
+ * couplings (interpolative bases), the matrix cores or the vector units for
+ * the general near and coupling products, and 16-, 32- or 64-row chunks for
+ * the upward products of levels with fewer than 1024 blocks. Every shape
+ * gives the same matvec up to rounding. best_ms: time per product of the
+ * kept shape. */
 int mxg_h2_autotune(mxg_h2_t h, int precision, long long nrhs, int reps, double* best_ms);
-/* The kept shape (near_sym / far_sym: 1 when the symmetric products run) and
- * setting one by hand (split_rows: a multiple of 4 in 8..256). */
-int mxg_h2_get_tuning(mxg_h2_t h, int precision, int* near_sym, int* far_sym, int* split_rows);
-int mxg_h2_set_tuning(mxg_h2_t h, int precision, int near_sym, int far_sym, int split_rows);
+/* The kept shape (near_sym / far_sym: 1 when the symmetric products run;
+ * matrix_cores: 1 when the general near and coupling products of aligned
+ * blocks run on the matrix cores) and setting one by hand (split_rows: a
+ * multiple of 4 in 8..256). */
+int mxg_h2_get_tuning(mxg_h2_t h, int precision, int* near_sym, int* far_sym, int* split_rows, int* matrix_cores);
+int mxg_h2_set_tuning(mxg_h2_t h, int precision, int near_sym, int far_sym, int split_rows, int matrix_cores);
 /* H^2 matvec y = A x on the device (matVecDeviceH2, device_csr_matrix.cu:328-338).
  * x, y: tree-ordered FP64 vectors of length nbodies, nrhs columns with leading
  * dimension ld. precision MXG_FP32 applies an FP32 copy of the operator data
@@ -262,11 +270,16 @@ enum {
   MXG_PROF_MATVEC_FP64 = 6,      /* a whole FP64 matvec, fork to join */
   MXG_PROF_MATVEC_FP32 = 7,      /* a whole FP32 matvec */
   MXG_PROF_ULV_SOLVE = 8,        /* a whole substitution (every level, both sweeps) */
-  MXG_PROF_CLASSES = 9
+  MXG_PROF_SKETCH = 9,           /* the far-field sketches of a level's basis construction */
+  MXG_PROF_CLASSES = 10
 };
 int mxg_profile_enable(mxg_ctx_t ctx, int enable);
 int mxg_profile_reset(mxg_ctx_t ctx);
 int mxg_profile_read(mxg_ctx_t ctx, int kernel_class, long long* launches, double* total_ms);
+/* Algorithmic work of the timed launches of a class since the last reset, for
+ * classes that count it (MXG_PROF_SKETCH: 2 M^2 nf flops per box of M columns
+ * and nf sketched far rows), else 0. */
+int mxg_profile_work(mxg_ctx_t ctx, int kernel_class, double* work);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,8 @@ GMRES_CGS2 = 1
 MXG_H2_FACTORABLE = 1
 # level data kinds of mxg_h2_download / mxg_h2_upload
 H2_NEAR, H2_BASIS, H2_R, H2_COUPLING, H2_SKELETON = 0, 1, 2, 3, 4
+# kernel timing classes of mxg_profile_read used from Python
+MXG_PROF_SKETCH = 9
 
 
 class MxgKernel(C.Structure):
@@ -94,7 +96,7 @@ def gpu() -> C.CDLL:
             "mxg_dense_matvec": [P, MxgKernel, P, c_ll, P, P],
             "mxg_dense_matvec_batch": [P, MxgKernel, P, c_ll, P, c_ll, P, c_ll, c_ll],
             "mxg_dense_matvec_batch_noalloc": [P, MxgKernel, P, c_ll, P, c_ll, P, c_ll, c_ll],
-            "mxg_block_gemv": [P, c_int, c_ll, c_ll, P, c_ll, P, c_ll, P, c_ll, c_ll],
+            "mxg_block_gemv": [P, c_int, c_ll, c_ll, P, c_ll, P, c_ll, P, c_ll, c_ll, c_int],
             "mxg_h2_create": [P, MxgKernel, P, c_ll, c_ll, c_int, C.POINTER(P)],
             "mxg_h2_destroy": [P],
             "mxg_h2_define_level": [P, c_ll, c_ll, P, P, P, P, P, P],
@@ -103,8 +105,8 @@ def gpu() -> C.CDLL:
             "mxg_ulv_destroy": [P],
             "mxg_h2_matvec": [P, P, P, c_ll, c_ll, c_int],
             "mxg_h2_autotune": [P, c_int, c_ll, c_int, C.POINTER(c_d)],
-            "mxg_h2_get_tuning": [P, c_int, C.POINTER(c_int), C.POINTER(c_int), C.POINTER(c_int)],
-            "mxg_h2_set_tuning": [P, c_int, c_int, c_int, c_int],
+            "mxg_h2_get_tuning": [P, c_int, C.POINTER(c_int), C.POINTER(c_int), C.POINTER(c_int), C.POINTER(c_int)],
+            "mxg_h2_set_tuning": [P, c_int, c_int, c_int, c_int, c_int],
             "mxg_h2_download": [P, c_ll, c_int, P, c_ll],
             "mxg_ulv_solve": [P, P, c_ll, c_ll],
             "mxg_ulv_info": [P, P],
@@ -112,6 +114,7 @@ def gpu() -> C.CDLL:
             "mxg_profile_enable": [P, c_int],
             "mxg_profile_reset": [P],
             "mxg_profile_read": [P, c_int, C.POINTER(c_ll), C.POINTER(c_d)],
+            "mxg_profile_work": [P, c_int, C.POINTER(c_d)],
         }
         for name, args in sigs.items():
             fn = getattr(L, name, None)

@@ -737,7 +737,7 @@ constexpr long long WAVE_JOB_BYTES = 65536;
 
 template <class T, class S>
 void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& tasks, const S* B, long long nB, const T* v,
-                      long long nv, long long ldv, T* out, long long nout, long long ldo) {
+                      long long nv, long long ldv, T* out, long long nout, long long ldo, bool matrix_cores) {
   MXG_REQUIRE(ldv >= nv && ldo >= nout, "right-hand-side strides shorter than the vectors");
   constexpr long long W = vwidth<S>();
   for (const PairTask& t : tasks) {
@@ -748,21 +748,23 @@ void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& t
     const int vec = ((reinterpret_cast<uintptr_t>(b) % 16) == 0 && t.ld % W == 0 && t.n % W == 0) ? 1 : 0;
     const long long wave_bytes = std::is_same<S, bf16_t>::value ? 2 * WAVE_JOB_BYTES : WAVE_JOB_BYTES;
     const int wave = (t.n <= 64 * (vec ? W : 1) && (long long)t.n * t.m * (long long)sizeof(S) <= wave_bytes) ? 1 : 0;
-    // the matrix cores take the aligned FP64 blocks; for FP32 and bfloat16
-    // blocks the vector-unit products measured faster (N = 2^20 sphere:
-    // near launch 1.89 against 2.22 ms, leaf substitution 0.82 against
-    // 1.76 ms), for FP64 the matrix cores (near launch 4.02 against 4.57 ms)
-    const int mx = (vec && sizeof(T) == 8 && std::is_same<T, S>::value) ? 1 : 0;
+    // the matrix cores take the aligned blocks stored in the compute type of
+    // the lists that ask for them: the matvec's near and coupling lists when
+    // its tuning selects them. The upward products and the substitution ran
+    // slower on them at N = 2^20 (FP64 substitution 4.32 against 3.69 ms,
+    // bfloat16 leaf substitution 1.76 against 0.82 ms) and stay on the
+    // vector units.
+    const int mx = (matrix_cores && vec && std::is_same<T, S>::value) ? 1 : 0;
     jobs.push_back(PairJob{b, v + t.v, out + t.out, ldv, ldo, t.n, t.m, t.ld, vec, wave, mx});
   }
 }
 template void pair_jobs_append<float, float>(std::vector<PairJob>&, const std::vector<PairTask>&, const float*, long long,
-                                             const float*, long long, long long, float*, long long, long long);
+                                             const float*, long long, long long, float*, long long, long long, bool);
 template void pair_jobs_append<double, double>(std::vector<PairJob>&, const std::vector<PairTask>&, const double*,
                                                long long, const double*, long long, long long, double*, long long,
-                                               long long);
+                                               long long, bool);
 template void pair_jobs_append<float, bf16_t>(std::vector<PairJob>&, const std::vector<PairTask>&, const bf16_t*,
-                                              long long, const float*, long long, long long, float*, long long, long long);
+                                              long long, const float*, long long, long long, float*, long long, long long, bool);
 
 void pair_list_upload(PairList& pl, const std::vector<PairJob>& jobs, hipStream_t s) {
   MXG_REQUIRE(jobs.size() < (1ull << 31), "pair list too long");
@@ -956,11 +958,11 @@ using namespace mxg;
 namespace {
 template <class T, class S>
 void one_block(mxg_ctx_t ctx, long long n, long long m, const void* B, long long ldb, const void* X, long long ldx, void* Y,
-               long long ldy, int nrhs) {
+               long long ldy, int nrhs, bool matrix_cores) {
   std::vector<PairJob> jobs;
   const std::vector<PairTask> t{PairTask{0, 0, 0, (int)n, (int)m, (int)ldb, 0}};
   pair_jobs_append<T, S>(jobs, t, static_cast<const S*>(B), (m - 1) * ldb + n, static_cast<const T*>(X), m, ldx,
-                         static_cast<T*>(Y), n, ldy);
+                         static_cast<T*>(Y), n, ldy, matrix_cores);
   PairList pl;
   pair_list_upload(pl, jobs, ctx->stream);
   pair_gemv<T, S>(pl, nrhs, ctx->stream);
@@ -969,7 +971,7 @@ void one_block(mxg_ctx_t ctx, long long n, long long m, const void* B, long long
 }  // namespace
 
 extern "C" int mxg_block_gemv(mxg_ctx_t ctx, int storage, long long n, long long m, const void* d_B, long long ldb,
-                              const void* d_X, long long ldx, void* d_Y, long long ldy, long long nrhs) {
+                              const void* d_X, long long ldx, void* d_Y, long long ldy, long long nrhs, int matrix_cores) {
   return guarded([&] {
     MXG_REQUIRE(ctx, "ctx is null");
     MXG_REQUIRE(n >= 1 && m >= 1 && n < (1 << 30) && m < (1 << 30), "block size out of range");
@@ -978,11 +980,11 @@ extern "C" int mxg_block_gemv(mxg_ctx_t ctx, int storage, long long n, long long
     MXG_REQUIRE(d_B && d_X && d_Y, "null pointer");
     MXG_HIP(hipSetDevice(ctx->device));
     if (storage == MXG_FP64)
-      one_block<double, double>(ctx, n, m, d_B, ldb, d_X, ldx, d_Y, ldy, (int)nrhs);
+      one_block<double, double>(ctx, n, m, d_B, ldb, d_X, ldx, d_Y, ldy, (int)nrhs, matrix_cores != 0);
     else if (storage == MXG_FP32)
-      one_block<float, float>(ctx, n, m, d_B, ldb, d_X, ldx, d_Y, ldy, (int)nrhs);
+      one_block<float, float>(ctx, n, m, d_B, ldb, d_X, ldx, d_Y, ldy, (int)nrhs, matrix_cores != 0);
     else if (storage == MXG_BF16)
-      one_block<float, bf16_t>(ctx, n, m, d_B, ldb, d_X, ldx, d_Y, ldy, (int)nrhs);
+      one_block<float, bf16_t>(ctx, n, m, d_B, ldb, d_X, ldx, d_Y, ldy, (int)nrhs, matrix_cores != 0);
     else
       throw Error(MXG_ERR_ARG, "unknown storage");
   });

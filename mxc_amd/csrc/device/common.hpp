@@ -156,6 +156,7 @@ struct mxg_ctx_s {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     long long launches = 0;
     double ms = 0.;
+    double work = 0.;  // algorithmic work of the launches, where counted
   };
   Timed timed[MXG_PROF_CLASSES];
   std::vector<hipEvent_t> event_pool;
@@ -183,7 +184,7 @@ struct ProfScope {
   int cls;
   hipStream_t st;
   hipEvent_t stop = nullptr;
-  ProfScope(mxg_ctx_s* ctx, int k, hipStream_t s = nullptr);
+  ProfScope(mxg_ctx_s* ctx, int k, hipStream_t s = nullptr, double work = 0.);
   ~ProfScope() { end(); }
   // records the stop event now (once; the destructor does it otherwise)
   void end();

@@ -61,8 +61,9 @@ void drain(mxg_ctx_s* c) {
 }
 }  // namespace
 
-ProfScope::ProfScope(mxg_ctx_s* ctx, int k, hipStream_t s) : c(ctx), cls(k), st(s ? s : ctx->stream) {
+ProfScope::ProfScope(mxg_ctx_s* ctx, int k, hipStream_t s, double work) : c(ctx), cls(k), st(s ? s : ctx->stream) {
   if (!c->profiling || cls < 0 || cls >= MXG_PROF_CLASSES) return;
+  c->timed[cls].work += work;
   hipEvent_t start = take_event(c);
   stop = take_event(c);
   MXG_HIP(hipEventRecord(start, st));
@@ -138,6 +139,7 @@ int mxg_profile_reset(mxg_ctx_t ctx) {
     for (auto& t : ctx->timed) {
       t.launches = 0;
       t.ms = 0.;
+      t.work = 0.;
     }
   });
 }
@@ -149,6 +151,14 @@ int mxg_profile_read(mxg_ctx_t ctx, int cls, long long* launches, double* total_
     drain(ctx);
     *launches = ctx->timed[cls].launches;
     *total_ms = ctx->timed[cls].ms;
+  });
+}
+
+int mxg_profile_work(mxg_ctx_t ctx, int cls, double* work) {
+  return guarded([&] {
+    MXG_REQUIRE(ctx && work, "null argument");
+    MXG_REQUIRE(cls >= 0 && cls < MXG_PROF_CLASSES, "kernel class out of range");
+    *work = ctx->timed[cls].work;
   });
 }
 

@@ -289,7 +289,7 @@ void construct_basis(mxg_h2_s* h, Level& Lv, long long l, const long long* far_o
       woff += 3 * M;
     }
     gen_tasks<double>(gens, h->bodies.p, h->k, s);
-    sketch_rows(sks, h->bodies.p, h->k, part.p, s);
+    sketch_rows(h->ctx, sks, h->bodies.p, h->k, part.p, s);
     qr_batched(qrs, s);
     cpqr_batched(cps, s);
     std::vector<int> rk(nb);
@@ -603,7 +603,7 @@ void ensure_matvec_jobs(mxg_h2_s* h) {
       else
         pair_jobs_append<T, T>(prod, pi ? Lv.far_t32 : Lv.far_t, Ops<T>::C(Lv),
                                pi ? (long long)Lv.C32.n : (long long)Lv.C.n, Ops<T>::Z(h, l), Lv.lenZ, Lv.lenZ,
-                               tb.p + Lv.tbaseF[pi], pi ? Lv.lenTF32 : Lv.lenTF, h->tlen[pi]);
+                               tb.p + Lv.tbaseF[pi], pi ? Lv.lenTF32 : Lv.lenTF, h->tlen[pi], tn.matrix_cores != 0);
     }
     pair_list_upload(Lv.up_pl[pi], up, s);
   }
@@ -614,7 +614,7 @@ void ensure_matvec_jobs(mxg_h2_s* h) {
   const bool near_sym = h->sym_near && tn.near_sym;
   if (!near_sym)
     pair_jobs_append<T, T>(near, Lf.near_t, Ops<T>::A(Lf), (long long)Lf.A.n, Ops<T>::x(h), h->n, h->n,
-                           tb.p + Lf.tbaseN[pi], Lf.lenTN, h->tlen[pi]);
+                           tb.p + Lf.tbaseN[pi], Lf.lenTN, h->tlen[pi], tn.matrix_cores != 0);
   pair_list_upload(h->near_pl[pi], near, s);
   std::vector<SymJob> sym;
   if (near_sym)
@@ -957,14 +957,16 @@ double autotune(mxg_h2_s* h, int nrhs, int reps) {
   std::vector<MatvecTuning> cand;
   for (int ns : {1, 0})
     for (int fs : {1, 0})
-      for (int rows : {16, 32, 64}) {
-        if ((ns && !h->sym_near) || (fs && !any_far)) continue;
-        MatvecTuning t;
-        t.near_sym = ns;
-        t.far_sym = fs;
-        t.split_rows = rows;
-        cand.push_back(t);
-      }
+      for (int mc : {1, 0})
+        for (int rows : {16, 32, 64}) {
+          if ((ns && !h->sym_near) || (fs && !any_far) || (mc && ns && fs)) continue;
+          MatvecTuning t;
+          t.near_sym = ns;
+          t.far_sym = fs;
+          t.split_rows = rows;
+          t.matrix_cores = mc;
+          cand.push_back(t);
+        }
   hipEvent_t e0, e1;
   MXG_HIP(hipEventCreate(&e0));
   MXG_HIP(hipEventCreate(&e1));
@@ -1232,9 +1234,9 @@ int mxg_h2_autotune(mxg_h2_t h, int precision, long long nrhs, int reps, double*
   });
 }
 
-int mxg_h2_get_tuning(mxg_h2_t h, int precision, int* near_sym, int* far_sym, int* split_rows) {
+int mxg_h2_get_tuning(mxg_h2_t h, int precision, int* near_sym, int* far_sym, int* split_rows, int* matrix_cores) {
   return guarded([&] {
-    MXG_REQUIRE(h && near_sym && far_sym && split_rows, "null argument");
+    MXG_REQUIRE(h && near_sym && far_sym && split_rows && matrix_cores, "null argument");
     MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32, "unknown precision");
     const MatvecTuning& t = h->tune[precision == MXG_FP64 ? 0 : 1];
     *near_sym = t.near_sym && h->sym_near;
@@ -1242,10 +1244,11 @@ int mxg_h2_get_tuning(mxg_h2_t h, int precision, int* near_sym, int* far_sym, in
     for (long long l = 0; l <= h->L; l++) any_far = any_far || h->lv[l].sym_far;
     *far_sym = t.far_sym && any_far;
     *split_rows = t.split_rows;
+    *matrix_cores = t.matrix_cores;
   });
 }
 
-int mxg_h2_set_tuning(mxg_h2_t h, int precision, int near_sym, int far_sym, int split_rows) {
+int mxg_h2_set_tuning(mxg_h2_t h, int precision, int near_sym, int far_sym, int split_rows, int matrix_cores) {
   return guarded([&] {
     MXG_REQUIRE(h, "null argument");
     MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32, "unknown precision");
@@ -1254,6 +1257,7 @@ int mxg_h2_set_tuning(mxg_h2_t h, int precision, int near_sym, int far_sym, int 
     h->tune[pi].near_sym = near_sym != 0;
     h->tune[pi].far_sym = far_sym != 0;
     h->tune[pi].split_rows = split_rows;
+    h->tune[pi].matrix_cores = matrix_cores != 0;
     h->jobs_ready[pi] = false;
   });
 }

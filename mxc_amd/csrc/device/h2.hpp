@@ -51,6 +51,7 @@ inline std::vector<PairTask> spread(const std::vector<PairTask>& tasks, size_t b
 struct MatvecTuning {
   int near_sym = 0, far_sym = 0;
   int split_rows = SPLIT_ROWS;
+  int matrix_cores = 0;  // general near and coupling products of aligned blocks on the matrix cores
 };
 
 // The device form of a task: absolute addresses, and vec = 1 when the block
@@ -210,10 +211,12 @@ void ulv_apply(mxg_ulv_s* f, double* x, long long ldx, long long nrhs);
 typedef unsigned short bf16_t;
 // Appends the jobs of tasks over the arrays B (nB elements, stored as S), v
 // (nv per right-hand side, ldv apart) and out (nout per right-hand side, ldo
-// apart), after checking every task against those sizes.
+// apart), after checking every task against those sizes. With matrix_cores,
+// blocks stored in the compute type that admit 16-byte column loads go to the
+// matrix-core kernel.
 template <class T, class S>
 void pair_jobs_append(std::vector<PairJob>& jobs, const std::vector<PairTask>& tasks, const S* B, long long nB, const T* v,
-                      long long nv, long long ldv, T* out, long long nout, long long ldo);
+                      long long nv, long long ldv, T* out, long long nout, long long ldo, bool matrix_cores = false);
 void pair_list_upload(PairList& pl, const std::vector<PairJob>& jobs, hipStream_t s);
 // out[r + k ldo] = sum_c B[r + c*ld] v[c + k ldv] for every job and the
 // right-hand sides k < nrhs (<= MAX_RHS), one workgroup per job, blocks

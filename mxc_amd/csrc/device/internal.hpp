@@ -123,8 +123,8 @@ struct SliceSum {
   int nsplit;
   int pad;
 };
-void sketch_rows(const std::vector<SketchBox>& boxes, const double* d_bodies, mxg_kernel_t k, double* d_partial,
-                 hipStream_t s);
+void sketch_rows(mxg_ctx_s* ctx, const std::vector<SketchBox>& boxes, const double* d_bodies, mxg_kernel_t k,
+                 double* d_partial, hipStream_t s);
 size_t sketch_partial_size(long long M, long long nf);
 
 // ---- misc kernels (h2.hip) ----

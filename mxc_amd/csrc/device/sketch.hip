@@ -38,6 +38,7 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
   __shared__ double Gs[SK_BK][SK_TJ + 1];
   __shared__ unsigned long long Ws[SK_BK][4];
   __shared__ double Xs[SK_TJ][3];
+  __shared__ double Xr[SK_BK][3];  // coordinates of the step's far rows
   const SketchTask tk = tasks[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
@@ -55,6 +56,19 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
     Xs[tid][1] = x1;
     Xs[tid][2] = x2;
   }
+  // The far rows' coordinates reach the evaluation through LDS, gathered by
+  // 3 SK_BK threads (row tid / 3, coordinate tid % 3) one step ahead, with
+  // the row indices two steps ahead, so that the gathers are in flight while
+  // the matrix cores run and no step waits on a dependent pair of loads.
+  constexpr int NX = 3 * SK_BK;
+  const int xr = tid / 3, xc = tid - 3 * (tid / 3);
+  int idx_next = -1;  // row index of this thread's entry one step ahead
+  if (tid < NX) {
+    const int t = tk.t0 + xr;
+    Xr[xr][xc] = t < tk.t1 ? bodies[3 * (long long)tk.rows[t] + xc] : 0.;
+    const int t1 = t + SK_BK;
+    idx_next = t1 < tk.t1 ? tk.rows[t1] : -1;
+  }
 
   d4 acc[4][4];
 #pragma unroll
@@ -68,17 +82,20 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
   const bool jvalid = tk.j0 + jj < tk.M;
 
   for (int tb = tk.t0; tb < tk.t1; tb += SK_BK) {
-    __syncthreads();
+    __syncthreads();  // Xr holds this step's rows; the previous step's reads of Gs are done
+    double xnext = 0.;
+    int idx_after = -1;
+    if (tid < NX) {
+      if (idx_next >= 0) xnext = bodies[3 * (long long)idx_next + xc];
+      const int t2 = tb + 2 * SK_BK + xr;
+      idx_after = t2 < tk.t1 ? tk.rows[t2] : -1;
+    }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int tt = (tid >> 6) + 4 * q;
       const int t = tb + tt;
       double v = 0.;
-      if (t < tk.t1 && jvalid) {
-        const long long r = tk.rows[t];
-        const double p[3] = {bodies[3 * r], bodies[3 * r + 1], bodies[3 * r + 2]};
-        v = kernel_eval(kid, diag, alpha, dist2(p, Xs[jj]));
-      }
+      if (t < tk.t1 && jvalid) v = kernel_eval(kid, diag, alpha, dist2(Xr[tt], Xs[jj]));
       Gs[tt][jj] = v;
     }
     if (tid < 4 * SK_BK) {
@@ -101,6 +118,11 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
           for (int bj = 0; bj < 4; bj++)
             acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], g[bj], acc[bi][bj], 0, 0, 0);
       }
+    }
+    // every read of Xr for this step came before the barrier above
+    if (tid < NX) {
+      Xr[xr][xc] = xnext;
+      idx_next = idx_after;
     }
   }
 
@@ -132,14 +154,16 @@ __global__ void __launch_bounds__(256) k_sum_slices(const SliceSum* __restrict__
 
 }  // namespace
 
-void sketch_rows(const std::vector<SketchBox>& boxes, const double* d_bodies, mxg_kernel_t k, double* d_partial,
-                 hipStream_t s) {
+void sketch_rows(mxg_ctx_s* ctx, const std::vector<SketchBox>& boxes, const double* d_bodies, mxg_kernel_t k,
+                 double* d_partial, hipStream_t s) {
   std::vector<SketchTask> tasks;
   std::vector<SliceSum> sums;
   std::vector<long long> prefix{0};
   size_t poff = 0;
+  double flops = 0.;
   for (const SketchBox& b : boxes) {
     if (b.M <= 0 || b.nf <= 0) continue;
+    flops += 2. * b.M * (double)b.M * (double)b.nf;
     const int nsplit = (int)((b.nf + SKETCH_ROWS - 1) / SKETCH_ROWS);
     const long long len = (long long)b.M * b.M;
     double* base = nsplit == 1 ? b.Y : d_partial + poff;
@@ -159,6 +183,7 @@ void sketch_rows(const std::vector<SketchBox>& boxes, const double* d_bodies, mx
   MXG_REQUIRE(tasks.size() < (1u << 31), "sketch grid too large");
   DevBuf<SketchTask> dt;
   dt.upload(tasks, s);
+  ProfScope prof(ctx, MXG_PROF_SKETCH, s, flops);
   hipLaunchKernelGGL(k_sketch, dim3((unsigned)tasks.size()), dim3(256), 0, s, dt.p, d_bodies, (int)k.id, k.diag, k.alpha);
   MXG_HIP(hipGetLastError());
   DevBuf<SliceSum> ds;
@@ -169,6 +194,7 @@ void sketch_rows(const std::vector<SketchBox>& boxes, const double* d_bodies, mx
     hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)prefix.back()), dim3(256), 0, s, ds.p, dp.p, (int)sums.size());
     MXG_HIP(hipGetLastError());
   }
+  prof.end();
   MXG_HIP(hipStreamSynchronize(s));
 }
 

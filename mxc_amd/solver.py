@@ -168,14 +168,15 @@ class H2:
         return ms.value, self.tuning(precision)
 
     def tuning(self, precision: int = N.MXG_FP64) -> dict:
-        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         dev = N.h2().mxc_solver_device(self.handle)
-        N.check_gpu(N.gpu().mxg_h2_get_tuning(dev, precision, C.byref(a), C.byref(b), C.byref(c)))
-        return {"near_symmetric": a.value, "far_symmetric": b.value, "split_rows": c.value}
+        N.check_gpu(N.gpu().mxg_h2_get_tuning(dev, precision, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return {"near_symmetric": a.value, "far_symmetric": b.value, "split_rows": c.value, "matrix_cores": d.value}
 
-    def set_tuning(self, precision: int, near_symmetric: int, far_symmetric: int, split_rows: int) -> None:
+    def set_tuning(self, precision: int, near_symmetric: int, far_symmetric: int, split_rows: int,
+                   matrix_cores: int = 0) -> None:
         dev = N.h2().mxc_solver_device(self.handle)
-        N.check_gpu(N.gpu().mxg_h2_set_tuning(dev, precision, near_symmetric, far_symmetric, split_rows))
+        N.check_gpu(N.gpu().mxg_h2_set_tuning(dev, precision, near_symmetric, far_symmetric, split_rows, matrix_cores))
 
     def factorize(self, precision: int = N.MXG_FP64) -> np.ndarray:
         info = np.zeros(self.levels + 1, np.int32)
