@@ -226,13 +226,17 @@ def main():
     bytes_prod_fp32 = 4 * (near_elems + 2 * nr * n)
     km = mixed["kernels"]["matvec_near_fp32"]
     achieved = bytes_prod_fp32 / (km["avg_ms"] * 1e-3) / 1e9 if km["avg_ms"] else None
-    sizes = []
-    hA = N.h2().mxc_solver_device(A.handle)
-    for l in range(L + 1):
-        d, r = A.dims(l)
-        sizes.append({"level": l, "boxes": len(d), "sum_dims": int(d.sum()), "sum_ranks": int(r.sum()),
-                      "near_elems": int(N.gpu().mxg_h2_level_size(hA, l, 0)),
-                      "coupling_elems": int(N.gpu().mxg_h2_level_size(hA, l, 3)), "basis_elems": int(np.sum(d * r))})
+    def level_sizes(op):
+        out, hd = [], N.h2().mxc_solver_device(op.handle)
+        for l in range(L + 1):
+            d, r = op.dims(l)
+            out.append({"level": l, "boxes": len(d), "sum_dims": int(d.sum()), "sum_ranks": int(r.sum()),
+                        "near_elems": int(N.gpu().mxg_h2_level_size(hd, l, 0)),
+                        "coupling_elems": int(N.gpu().mxg_h2_level_size(hd, l, 3)), "basis_elems": int(np.sum(d * r))})
+        return out
+
+    sizes = level_sizes(A)
+    sizes_in = level_sizes(A_in) if A_in is not None else None
 
     value = args.steps * args.batch / mixed["elapsed"]
     value64 = args.steps * args.batch / fp64["elapsed"]
@@ -293,6 +297,7 @@ def main():
         "kernels_fp64": fp64["kernels"],
         "matvec_autotune": {k: {"ms": round(v[0], 4), **v[1]} for k, v in tuning.items()},
         "operator_levels": sizes,
+        "inner_operator_levels": sizes_in,
         "roofline": {
             "kernel": f"k_pair_gemv<float, float, {nr}>: the matvec's launch of the leaf near products (FP32)",
             "bound": "hbm",

@@ -140,6 +140,17 @@ int mxg_h2_construct_level(mxg_h2_t h, long long level, long long nboxes, const 
                            const long long* far_rp, const long long* far_col, const long long* far_off,
                            const long long* far_idx, double tol, long long maxrank, int orth,
                            long long* ranks_out);
+/* The same with each box's far-body list given as runs of consecutive body
+ * indices, in list order: run k covers bodies far_runs[2k] ..
+ * far_runs[2k] + far_runs[2k + 1] - 1, box i owns runs far_run_off[i] ..
+ * far_run_off[i + 1] - 1. A far cell kept whole is one run, so the list of
+ * the reference (h-matrix.cpp:121-127) crosses the bus in a few runs per cell
+ * and is expanded on the device. */
+int mxg_h2_construct_level_runs(mxg_h2_t h, long long level, long long nboxes, const long long* dims,
+                                const long long* leaf_begin, const long long* near_rp, const long long* near_col,
+                                const long long* far_rp, const long long* far_col, const long long* far_run_off,
+                                const long long* far_runs, double tol, long long maxrank, int orth,
+                                long long* ranks_out);
 
 /* Copy of level data to the host (copyDataOutMatrixDesc, device_precondition.cu:204-208).
  * what: 0 = near blocks A, 1 = bases Q (M*M per box), 2 = R (M*M per box), 3 = couplings C,

@@ -67,6 +67,8 @@ typedef struct {
   long long sampling;  /* far bodies per box, <= 0 for all */
   int factorable;      /* keep what mxc_solver_factorize needs */
   int householder_rows; /* reduce far rows by the Householder R factor (MXG_H2_HOUSEHOLDER_ROWS) */
+  int explicit_far;     /* hand the far lists over as one index per body (mxg_h2_construct_level)
+                           instead of runs of consecutive bodies (mxg_h2_construct_level_runs) */
 } mxc_h2_opts_t;
 typedef struct mxc_solver_s* mxc_solver_t;
 int mxc_solver_create(mxc_problem_t p, const mxc_h2_opts_t* o, mxc_solver_t* s);

@@ -54,6 +54,7 @@ class MxcH2Opts(C.Structure):
         ("sampling", c_ll),
         ("factorable", c_int),
         ("householder_rows", c_int),
+        ("explicit_far", c_int),
     ]
 
 

@@ -207,23 +207,119 @@ namespace {
 // Budget of doubles for the far-sample matrices G of one construction chunk.
 constexpr size_t G_BUDGET = size_t(1) << 28;
 
-void construct_basis(mxg_h2_s* h, Level& Lv, long long l, const long long* far_off, const long long* far_idx, double tol,
-                     long long maxrank, int orth) {
+// The far-body rows of a level's boxes on the device: box i's rows are
+// d[off[i] .. off[i] + nf[i]).
+struct FarRows {
+  std::vector<long long> nf, off;
+  DevBuf<int> d;
+};
+
+// From explicit index lists: far_off[nb + 1], far_idx[far_off[nb]].
+void far_rows_explicit(mxg_h2_s* h, long long nb, const long long* far_off, const long long* far_idx, FarRows& fr) {
+  fr.nf.assign(nb, 0);
+  fr.off.assign(nb + 1, 0);
+  if (!far_off) return;
+  for (long long i = 0; i < nb; i++) {
+    fr.nf[i] = far_off[i + 1] - far_off[i];
+    MXG_REQUIRE(fr.nf[i] >= 0, "far offsets must not decrease");
+    fr.off[i + 1] = far_off[i + 1] - far_off[0];
+  }
+  const long long nfar = fr.off[nb];
+  if (nfar == 0) return;
+  std::vector<int> fi(nfar);
+  const long long* src = far_idx + far_off[0];
+  for (long long t = 0; t < nfar; t++) {
+    MXG_REQUIRE(src[t] >= 0 && src[t] < h->n, "far body index out of range");
+    fi[t] = (int)src[t];
+  }
+  fr.d.upload(fi, h->ctx->stream);
+  MXG_HIP(hipStreamSynchronize(h->ctx->stream));
+}
+
+// Box b's far rows from its runs of consecutive body indices: runs are taken
+// 256 at a time, their counts scanned in LDS, and the workgroup writes the
+// rows in order, each thread finding its row's run by binary search.
+__global__ void __launch_bounds__(256) k_expand_runs(const int2* __restrict__ runs, const long long* __restrict__ run_off,
+                                                     const long long* __restrict__ row_off, int* __restrict__ rows) {
+  __shared__ long long scan[256];
+  __shared__ int first[256];
+  const long long b = blockIdx.x;
+  const long long r0 = run_off[b], r1 = run_off[b + 1];
+  const int tid = threadIdx.x;
+  long long base = row_off[b];
+  for (long long rb = r0; rb < r1; rb += 256) {
+    const long long r = rb + tid;
+    const int2 run = r < r1 ? runs[r] : make_int2(0, 0);
+    first[tid] = run.x;
+    scan[tid] = run.y;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      const long long v = tid >= o ? scan[tid - o] : 0;
+      __syncthreads();
+      scan[tid] += v;
+      __syncthreads();
+    }
+    const long long tot = scan[255];
+    for (long long e = tid; e < tot; e += 256) {
+      int lo = 0, hi = 255;  // first run whose inclusive count exceeds e
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (scan[mid] > e) hi = mid;
+        else lo = mid + 1;
+      }
+      rows[base + e] = first[lo] + (int)(e - (lo ? scan[lo - 1] : 0));
+    }
+    base += tot;
+    __syncthreads();  // the batch's scan and firsts are read before the next batch overwrites them
+  }
+}
+
+// From runs: box i's rows are the runs run_off[i] .. run_off[i + 1] - 1 in
+// order, run k covering bodies runs[2k] .. runs[2k] + runs[2k + 1] - 1.
+void far_rows_runs(mxg_h2_s* h, long long nb, const long long* run_off, const long long* runs, FarRows& fr) {
+  hipStream_t s = h->ctx->stream;
+  fr.nf.assign(nb, 0);
+  fr.off.assign(nb + 1, 0);
+  if (!run_off) return;
+  const long long r_base = run_off[0], nruns = run_off[nb] - r_base;
+  std::vector<int2> packed(std::max<long long>(nruns, 1));
+  std::vector<long long> roff(nb + 1);
+  for (long long i = 0; i < nb; i++) {
+    MXG_REQUIRE(run_off[i + 1] >= run_off[i], "run offsets must not decrease");
+    long long rows = 0;
+    for (long long k = run_off[i]; k < run_off[i + 1]; k++) {
+      const long long f = runs[2 * k], c = runs[2 * k + 1];
+      MXG_REQUIRE(c >= 0 && f >= 0 && f + c <= h->n, "far run out of range");
+      packed[k - r_base] = make_int2((int)f, (int)c);
+      rows += c;
+    }
+    fr.nf[i] = rows;
+    fr.off[i + 1] = fr.off[i] + rows;
+    roff[i] = run_off[i] - r_base;
+  }
+  roff[nb] = nruns;
+  const long long nfar = fr.off[nb];
+  if (nfar == 0) return;
+  MXG_REQUIRE(nfar < (1ll << 40), "far lists too long");
+  DevBuf<int2> d_runs;
+  DevBuf<long long> d_roff, d_off;
+  d_runs.upload(packed, s);
+  d_roff.upload(roff, s);
+  d_off.upload(fr.off, s);
+  fr.d.alloc(nfar);
+  hipLaunchKernelGGL(k_expand_runs, dim3((unsigned)nb), dim3(256), 0, s, d_runs.p, d_roff.p, d_off.p, fr.d.p);
+  MXG_HIP(hipGetLastError());
+  MXG_HIP(hipStreamSynchronize(s));
+}
+
+void construct_basis(mxg_h2_s* h, Level& Lv, long long l, const FarRows& fr, double tol, long long maxrank, int orth) {
   hipStream_t s = h->ctx->stream;
   const long long nb = Lv.nb;
   std::vector<long long> nf(nb, 0);
-  if (l > 0 && far_off)
-    for (long long i = 0; i < nb; i++) nf[i] = far_off[i + 1] - far_off[i];
-  long long nfar = (l > 0 && far_off) ? far_off[nb] : 0;
-  DevBuf<int> d_far;
-  if (nfar > 0) {
-    std::vector<int> fi(nfar);
-    for (long long t = 0; t < nfar; t++) {
-      MXG_REQUIRE(far_idx[t] >= 0 && far_idx[t] < h->n, "far body index out of range");
-      fi[t] = (int)far_idx[t];
-    }
-    d_far.upload(fi, s);
-  }
+  if (l > 0)
+    for (long long i = 0; i < nb; i++) nf[i] = fr.nf[i];
+  const int* d_far = fr.d.p;
+  const std::vector<long long>& far_off = fr.off;
   DevBuf<int> d_perm, d_rank, d_stmp;
   d_perm.alloc(std::max<long long>(Lv.lenX, 1));
   d_stmp.alloc(std::max<long long>(Lv.lenX, 1));
@@ -277,9 +373,9 @@ void construct_basis(mxg_h2_s* h, Level& Lv, long long l, const long long* far_o
       if (K <= 0) continue;
       double* Gi = G.p + goff[i - b0];
       if (sketch && nf[i] > M) {
-        sks.push_back(SketchBox{d_far.p + far_off[i], Lv.S.p + Lv.offX[i], Gi, nf[i], (int)M, 0});
+        sks.push_back(SketchBox{d_far + far_off[i], Lv.S.p + Lv.offX[i], Gi, nf[i], (int)M, 0});
       } else {
-        gens.push_back(GenTask{d_far.p + far_off[i], Lv.S.p + Lv.offX[i], 0, 0, Gi, (int)nf[i], (int)M, (int)nf[i], 0});
+        gens.push_back(GenTask{d_far + far_off[i], Lv.S.p + Lv.offX[i], 0, 0, Gi, (int)nf[i], (int)M, (int)nf[i], 0});
         if (K < nf[i]) qrs.push_back(QrTask{Gi, tau.p + toff, (int)nf[i], (int)M, (int)nf[i], 1});
       }
       toff += K;
@@ -1044,11 +1140,14 @@ int mxg_h2_destroy(mxg_h2_t h) {
   });
 }
 
-int mxg_h2_construct_level(mxg_h2_t h, long long l, long long nb, const long long* dims, const long long* leaf_begin,
-                           const long long* near_rp, const long long* near_col, const long long* far_rp,
-                           const long long* far_col, const long long* far_off, const long long* far_idx, double tol,
-                           long long maxrank, int orth, long long* ranks_out) {
-  return guarded([&] {
+}  // extern "C"
+
+namespace mxg {
+namespace {
+void construct_level(mxg_h2_t h, long long l, long long nb, const long long* dims, const long long* leaf_begin,
+                     const long long* near_rp, const long long* near_col, const long long* far_rp, const long long* far_col,
+                     const FarRows& fr, double tol, long long maxrank, int orth, long long* ranks_out) {
+  {
     MXG_REQUIRE(h && dims && near_rp && far_rp && ranks_out, "null argument");
     MXG_REQUIRE(l >= 0 && l <= h->L, "level out of range");
     MXG_REQUIRE(l < h->L || leaf_begin, "leaf level needs leaf_begin");
@@ -1099,11 +1198,45 @@ int mxg_h2_construct_level(mxg_h2_t h, long long l, long long nb, const long lon
       copy2d_batched<double>(qs, qd, qm, qn, qls, qld, s);
       link_children(h, Lv, *low, true);
     }
-    construct_basis(h, Lv, l, far_off, far_idx, tol, maxrank, orth);
+    construct_basis(h, Lv, l, fr, tol, maxrank, orth);
     for (long long i = 0; i < nb; i++) ranks_out[i] = Lv.ranks[i];
     level_ranks(h, Lv);
     construct_coupling(h, Lv, orth);
     level_finish(h, Lv, !low);
+  }
+}
+}  // namespace
+}  // namespace mxg
+
+extern "C" {
+
+int mxg_h2_construct_level(mxg_h2_t h, long long l, long long nb, const long long* dims, const long long* leaf_begin,
+                           const long long* near_rp, const long long* near_col, const long long* far_rp,
+                           const long long* far_col, const long long* far_off, const long long* far_idx, double tol,
+                           long long maxrank, int orth, long long* ranks_out) {
+  return guarded([&] {
+    MXG_REQUIRE(h && dims && ranks_out, "null argument");
+    MXG_REQUIRE(l >= 0 && l <= h->L, "level out of range");
+    MXG_HIP(hipSetDevice(h->ctx->device));
+    FarRows fr;
+    if (l > 0) far_rows_explicit(h, nb, far_off, far_idx, fr);
+    else far_rows_explicit(h, nb, nullptr, nullptr, fr);
+    construct_level(h, l, nb, dims, leaf_begin, near_rp, near_col, far_rp, far_col, fr, tol, maxrank, orth, ranks_out);
+  });
+}
+
+int mxg_h2_construct_level_runs(mxg_h2_t h, long long l, long long nb, const long long* dims, const long long* leaf_begin,
+                                const long long* near_rp, const long long* near_col, const long long* far_rp,
+                                const long long* far_col, const long long* far_run_off, const long long* far_runs,
+                                double tol, long long maxrank, int orth, long long* ranks_out) {
+  return guarded([&] {
+    MXG_REQUIRE(h && dims && ranks_out, "null argument");
+    MXG_REQUIRE(l >= 0 && l <= h->L, "level out of range");
+    MXG_REQUIRE(!far_run_off || far_runs || far_run_off[nb] == far_run_off[0], "null far runs");
+    MXG_HIP(hipSetDevice(h->ctx->device));
+    FarRows fr;
+    far_rows_runs(h, nb, l > 0 ? far_run_off : nullptr, far_runs, fr);
+    construct_level(h, l, nb, dims, leaf_begin, near_rp, near_col, far_rp, far_col, fr, tol, maxrank, orth, ranks_out);
   });
 }
 

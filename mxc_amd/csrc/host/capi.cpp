@@ -201,6 +201,7 @@ int mxc_solver_create(mxc_problem_t p, const mxc_h2_opts_t* o, mxc_solver_t* out
     b.hss_far = o->hss_far != 0;
     b.sampling = o->sampling;
     b.householder_rows = o->householder_rows != 0;
+    b.explicit_far = o->explicit_far != 0;
     auto s = std::make_unique<mxc_solver_s>();
     s->p = p;
     s->s = mxc::H2MatrixSolver(p->ctx, *acc, b, p->cells, o->theta, p->bodies.data(), p->levels, o->factorable != 0);

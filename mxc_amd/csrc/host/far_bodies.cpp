@@ -36,8 +36,22 @@ std::vector<long long> ancestry(long long cell) {
 
 }  // namespace
 
-std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR& basis_far, const CSR& near,
-                                       const CSR& far, bool with_near, long long cell, long long sampling) {
+namespace {
+// appends bodies [first, first + count) to a run list, extending its last run
+// when the two are adjacent
+void add_run(std::vector<long long>& out, long long first, long long count) {
+  if (count <= 0) return;
+  const size_t n = out.size();
+  if (n >= 2 && out[n - 2] + out[n - 1] == first) out[n - 1] += count;
+  else {
+    out.push_back(first);
+    out.push_back(count);
+  }
+}
+}  // namespace
+
+void sampleFarRuns(const std::vector<Cell>& cells, const CSR& basis_far, const CSR& near, const CSR& far, bool with_near,
+                   long long cell, long long sampling, std::vector<long long>& out) {
   const std::vector<long long> path = ancestry(cell);
   long long total = 0;
   for (long long a : path)
@@ -45,15 +59,13 @@ std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR
       const Cell& f = cells[basis_far.ColIndex[k]];
       total += f.Body[1] - f.Body[0];
     }
-  std::vector<long long> out;
   if (sampling <= 0 || total <= sampling) {
-    out.reserve(total);
     for (long long a : path)
       for (long long k = basis_far.RowIndex[a]; k < basis_far.RowIndex[a + 1]; k++) {
         const Cell& f = cells[basis_far.ColIndex[k]];
-        for (long long b = f.Body[0]; b < f.Body[1]; b++) out.push_back(b);
+        add_run(out, f.Body[0], f.Body[1] - f.Body[0]);
       }
-    return out;
+    return;
   }
 
   // shells, closest first
@@ -85,11 +97,12 @@ std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR
     for (size_t k = 0; k < sh.size(); k++) sh[k].quota = (k < resolved) ? sh[k].count : std::min(sh[k].count, sampling);
   }
 
-  long long kept = 0;
-  for (const Shell& s : sh) kept += s.quota;
-  out.reserve(kept);
   for (size_t k = sh.size(); k-- > 0;) {
     const Shell& s = sh[k];
+    if (s.quota == s.count) {  // positions floor(t n / n) = t: every range whole
+      for (const auto& r : s.ranges) add_run(out, r.first, r.second - r.first);
+      continue;
+    }
     size_t r = 0;
     long long before = 0;  // bodies in ranges[0..r)
     for (long long t = 0; t < s.quota; t++) {
@@ -98,9 +111,21 @@ std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR
         before += s.ranges[r].second - s.ranges[r].first;
         r++;
       }
-      out.push_back(s.ranges[r].first + (pos - before));
+      add_run(out, s.ranges[r].first + (pos - before), 1);
     }
   }
+}
+
+std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR& basis_far, const CSR& near,
+                                       const CSR& far, bool with_near, long long cell, long long sampling) {
+  std::vector<long long> runs;
+  sampleFarRuns(cells, basis_far, near, far, with_near, cell, sampling, runs);
+  long long total = 0;
+  for (size_t k = 1; k < runs.size(); k += 2) total += runs[k];
+  std::vector<long long> out;
+  out.reserve(total);
+  for (size_t k = 0; k < runs.size(); k += 2)
+    for (long long b = runs[k]; b < runs[k] + runs[k + 1]; b++) out.push_back(b);
   return out;
 }
 
@@ -114,6 +139,23 @@ FarBodies::FarBodies(const std::vector<Cell>& cells, const CSR& basis_far, const
   for (long long i = 0; i < nb; i++) offsets[i + 1] = offsets[i] + (long long)lists[i].size();
   indices.resize(offsets[nb]);
   for (long long i = 0; i < nb; i++) std::copy(lists[i].begin(), lists[i].end(), indices.begin() + offsets[i]);
+}
+
+FarRuns::FarRuns(const std::vector<Cell>& cells, const CSR& basis_far, const CSR& near, const CSR& far, bool with_near,
+                 long long level, long long sampling) {
+  const long long nb = 1ll << level, first = nb - 1;
+  std::vector<std::vector<long long>> lists(nb);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long long i = 0; i < nb; i++) sampleFarRuns(cells, basis_far, near, far, with_near, first + i, sampling, lists[i]);
+  offsets.assign(nb + 1, 0);
+  std::vector<size_t> at(nb + 1, 0);
+  for (long long i = 0; i < nb; i++) {
+    offsets[i + 1] = offsets[i] + (long long)lists[i].size() / 2;
+    at[i + 1] = at[i] + lists[i].size();
+  }
+  runs.resize(at[nb]);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long long i = 0; i < nb; i++) std::copy(lists[i].begin(), lists[i].end(), runs.begin() + at[i]);
 }
 
 }  // namespace mxc

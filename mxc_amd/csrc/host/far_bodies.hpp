@@ -55,7 +55,25 @@ class FarBodies {
   const long long* fbodies_at_i(long long i) const { return indices.data() + offsets[i]; }
 };
 
-// Sampled far-body list of one cell (exposed for tests).
+// The same lists as runs of consecutive body indices, in list order: run k
+// covers bodies runs[2k] .. runs[2k] + runs[2k + 1] - 1. A shell kept whole is
+// one run per cell, so a level's lists take a few runs per far cell instead
+// of one index per body; the device expands them
+// (mxg_h2_construct_level_runs).
+class FarRuns {
+ public:
+  std::vector<long long> offsets;  // nboxes + 1, into the runs
+  std::vector<long long> runs;     // 2 per run: first body, count
+
+  FarRuns() = default;
+  FarRuns(const std::vector<Cell>& cells, const CSR& basis_far, const CSR& near, const CSR& far, bool with_near,
+          long long level, long long sampling);
+};
+
+// Sampled far-body list of one cell as runs (first body, count) appended to
+// out, and the list itself (its runs expanded).
+void sampleFarRuns(const std::vector<Cell>& cells, const CSR& basis_far, const CSR& near, const CSR& far, bool with_near,
+                   long long cell, long long sampling, std::vector<long long>& out);
 std::vector<long long> sampleFarBodies(const std::vector<Cell>& cells, const CSR& basis_far, const CSR& near,
                                        const CSR& far, bool with_near, long long cell, long long sampling);
 

@@ -46,11 +46,24 @@ void H2Matrix::construct(mxg_h2_t dev, const BasisOptions& opt, long long maxran
   for (long long d : Dims) lenX += d;
   LowerZ = 0;
 
-  FarBodies fb;
-  if (lvl > 0) fb = FarBodies(cells, BasisFar, Near, Far, opt.hss_far, lvl, opt.sampling);
-  check(mxg_h2_construct_level(dev, lvl, nboxes, Dims.data(), lower ? nullptr : leaf_begin.data(), ARows.data(),
-                               ACols.data(), CRows.data(), CCols.data(), lvl > 0 ? fb.offsets.data() : nullptr,
-                               lvl > 0 ? fb.indices.data() : nullptr, opt.tol, maxrank, opt.orth ? 1 : 0, DimsLr.data()));
+  const long long* lb = lower ? nullptr : leaf_begin.data();
+  if (opt.explicit_far) {
+    FarBodies fb;
+    if (lvl > 0) fb = FarBodies(cells, BasisFar, Near, Far, opt.hss_far, lvl, opt.sampling);
+    check(mxg_h2_construct_level(dev, lvl, nboxes, Dims.data(), lb, ARows.data(), ACols.data(), CRows.data(),
+                                 CCols.data(), lvl > 0 ? fb.offsets.data() : nullptr,
+                                 lvl > 0 ? fb.indices.data() : nullptr, opt.tol, maxrank, opt.orth ? 1 : 0,
+                                 DimsLr.data()));
+  } else {
+    // runs of consecutive bodies: a few per far cell instead of one index per
+    // body (2.9e9 far rows over the levels of the N = 2^20 sphere operator)
+    FarRuns fr;
+    if (lvl > 0) fr = FarRuns(cells, BasisFar, Near, Far, opt.hss_far, lvl, opt.sampling);
+    check(mxg_h2_construct_level_runs(dev, lvl, nboxes, Dims.data(), lb, ARows.data(), ACols.data(), CRows.data(),
+                                      CCols.data(), lvl > 0 ? fr.offsets.data() : nullptr,
+                                      lvl > 0 ? fr.runs.data() : nullptr, opt.tol, maxrank, opt.orth ? 1 : 0,
+                                      DimsLr.data()));
+  }
 }
 
 }  // namespace mxc

@@ -22,6 +22,7 @@ struct BasisOptions {
   bool hss_far = false;       // basis far lists from the sibling (theta = 0) lists (solver.cpp:32)
   long long sampling = 0;     // far bodies per box (<= 0: all)
   bool householder_rows = false;  // far rows reduced by the Householder R factor, not the sketch
+  bool explicit_far = false;      // far lists as one index per body instead of runs
 };
 
 class H2Matrix {

@@ -111,11 +111,12 @@ class H2:
     """An H^2 matrix on the device (H2MatrixSolver)."""
 
     def __init__(self, problem: Problem, kernel="laplace", diag=1e3, alpha=1.0, theta=2.0, tol=0.0, rank=0,
-                 leveled_rank=0, orth=False, hss_far=False, sampling=0, factorable=False, householder_rows=False):
+                 leveled_rank=0, orth=False, hss_far=False, sampling=0, factorable=False, householder_rows=False,
+                 explicit_far=False):
         self.problem = problem
         self.kernel = (kernel, diag, alpha)
         o = N.MxcH2Opts(N.KERNELS[kernel], diag, alpha, theta, tol, rank, leveled_rank, int(orth), int(hss_far), sampling,
-                        int(factorable), int(householder_rows))
+                        int(factorable), int(householder_rows), int(explicit_far))
         h = C.c_void_p()
         N.check_h2(N.h2().mxc_solver_create(problem.handle, C.byref(o), C.byref(h)))
         self.handle = h.value
@@ -123,9 +124,11 @@ class H2:
         problem.ctx.adopt(self, Context.SOLVERS)
 
     @classmethod
-    def operator(cls, problem, eps, sampling, kernel="laplace", diag=1e3, alpha=1.0, theta=2.0, householder_rows=False):
+    def operator(cls, problem, eps, sampling, kernel="laplace", diag=1e3, alpha=1.0, theta=2.0, householder_rows=False,
+                 explicit_far=False):
         """The accurate operator of lorasp: interpolative bases at tolerance eps (solver.cpp:28-36, epi < 1)."""
-        return cls(problem, kernel, diag, alpha, theta, tol=eps, sampling=sampling, householder_rows=householder_rows)
+        return cls(problem, kernel, diag, alpha, theta, tol=eps, sampling=sampling, householder_rows=householder_rows,
+                   explicit_far=explicit_far)
 
     @classmethod
     def preconditioner(cls, problem, rank, sampling, kernel="laplace", diag=1e3, alpha=1.0, theta=2.0, eps=0.0,
