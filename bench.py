@@ -58,6 +58,9 @@ def parse():
                          "serve: 16 resident problems under a Poisson request stream")
     ap.add_argument("--rate", type=float, default=10.0, help="serve: offered load, requests per second")
     ap.add_argument("--requests", type=int, default=160, help="serve: requests of the timed stream")
+    ap.add_argument("--problem", choices=["sphere-yukawa", "cube-laplace"], default="sphere-yukawa",
+                    help="sphere-yukawa: the metric's workload (points uniform on the unit sphere, exp(-r)/r); "
+                         "cube-laplace: points uniform in the unit cube, 1/r (K(0) = 1e3 for both)")
     ap.add_argument("--cpu-phases", action="store_true",
                     help="only time the oracle's phases on the host cores at N=20000 and N=65536 (no GPU work)")
     return ap.parse_args()
@@ -116,11 +119,12 @@ def main():
     from mxc_amd.solver import H2, Problem, generate_points, uniform_vector
 
     n, leaf, theta = args.n, 256, 2.0
-    kernel, diag, alpha = "yukawa", 1e3, 1.0
+    cube = args.problem == "cube-laplace"
+    kernel, diag, alpha = ("laplace" if cube else "yukawa"), 1e3, 1.0
     tol, restart, max_cycles = 1e-10, 30, 20
     ctx = Context(0)
     t0 = time.time()
-    pts = generate_points("sphere", n, 2)
+    pts = generate_points("cube" if cube else "sphere", n, 2)
     prob = Problem(ctx, pts, leaf)
     # the far-field sketches of the operator's construction are timed with
     # HIP events (one pair per level) and their flops counted
@@ -255,8 +259,9 @@ def main():
                  " + f32 inner operator, f64 Arnoldi/residual",
         "data": "synthetic (seeded points and right-hand sides)",
         "config": {
-            "workload": "H2 GMRES solve, unit-sphere Yukawa N=2^20, leaf 256",
-            "N": n, "leaf": leaf, "theta": theta, "kernel": "yukawa exp(-r)/r, K(0)=1e3",
+            "workload": ("H2 GMRES solve, unit-cube Laplace" if cube else "H2 GMRES solve, unit-sphere Yukawa") +
+                        f" N={n}, leaf 256",
+            "N": n, "leaf": leaf, "theta": theta, "kernel": "laplace 1/r, K(0)=1e3" if cube else "yukawa exp(-r)/r, K(0)=1e3",
             "operator_eps": args.op_eps, "inner_operator_eps": args.inner_eps, "sampling": args.sampling,
             "sampling_rule": "operators: the max(3, ceil(L/2)) nearest far shells of a box are kept whole, every "
                              "farther shell keeps at most `sampling` bodies; preconditioner: `sampling` bodies in "
@@ -314,7 +319,7 @@ def main():
     ctx.close()
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.sampling, args.op_eps, n)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.sampling, args.op_eps, n, cube)
     print(json.dumps(out))
 
 
@@ -499,7 +504,7 @@ def timed_solves(A, M, O, ns, seconds, max_solves):
     return float(np.mean(times)), k
 
 
-def cpu_baseline(ns, sampling, op_eps, n_target):
+def cpu_baseline(ns, sampling, op_eps, n_target, cube=False):
     """The CPU oracle (FP64 restatement of the reference) on the metric's
     workload at ns points (construction and factorization untimed): GMRES
     solves (reference mode, restart 30, tol 1e-10) timed for about 10 s on
@@ -510,10 +515,11 @@ def cpu_baseline(ns, sampling, op_eps, n_target):
     import oracle_lib as O
 
     threads = host_threads()
-    pts = O.sphere_points(ns, 2)
+    pts = O.cube_points(ns, 2) if cube else O.sphere_points(ns, 2)
+    kern = "laplace" if cube else "yukawa"
     P = O.Problem(pts, 256)
-    A = P.h2(kernel="yukawa", alpha=1.0, tol=op_eps, sampling=sampling, nthreads=threads)
-    M = P.h2(kernel="yukawa", alpha=1.0, tol=1e-6, rank=64, orth=True, hss_far=True, sampling=sampling, nthreads=threads)
+    A = P.h2(kernel=kern, alpha=1.0, tol=op_eps, sampling=sampling, nthreads=threads)
+    M = P.h2(kernel=kern, alpha=1.0, tol=1e-6, rank=64, orth=True, hss_far=True, sampling=sampling, nthreads=threads)
     M.factorize()
     per, k = timed_solves(A, M, O, ns, 10.0, 200)
     A.set_threads(1)
@@ -524,7 +530,8 @@ def cpu_baseline(ns, sampling, op_eps, n_target):
         "unit": "solves/s (scaled to N=2^20 from the measured rate at the sample size)",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle GMRES solves (reference mode, CGS1, restart 30, tol 1e-10) at N={ns}, unit-sphere Yukawa, "
+        "sample": f"oracle GMRES solves (reference mode, CGS1, restart 30, tol 1e-10) at N={ns}, "
+                  f"{'unit-cube Laplace' if cube else 'unit-sphere Yukawa'}, "
                   f"operator eps {op_eps:g}, preconditioner eps 1e-6 rank 64: {k} solves on {threads} threads "
                   f"({per * 1e3:.1f} ms each), {k1} on 1 thread ({per1 * 1e3:.1f} ms each)",
         "measured_solves_per_s_at_sample": round(1.0 / per, 4),

@@ -1,6 +1,9 @@
 #include "solver.hpp"
 
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
@@ -42,10 +45,23 @@ void H2MatrixSolver::build(const MatrixAccessor& eval, const std::vector<Cell>& 
   const int flags = (factorable ? MXG_H2_FACTORABLE : 0) | (opts.householder_rows ? MXG_H2_HOUSEHOLDER_ROWS : 0);
   check(mxg_h2_create(ctx, eval.desc(), bodies, nbodies, levels, flags, &dev));
   A.assign(levels + 1, H2Matrix());
+  // MXC_PROGRESS set: one line per constructed level on stderr
+  const bool progress = std::getenv("MXC_PROGRESS") != nullptr;
   for (long long l = levels; l >= 0; l--) {
+    const auto t0 = std::chrono::steady_clock::now();
     // rank_func(l) = (levels - l) * leveled_rank + rank (solver.cpp:29)
     long long maxrank = opts.rank > 0 ? (levels - l) * opts.leveled_rank + opts.rank : 0;
     A[l].construct(dev, opts, maxrank, cells, Near, Far, BasisFar, l, l < levels ? &A[l + 1] : nullptr);
+    if (progress) {
+      long long rsum = 0, rmax = 0;
+      for (long long r : A[l].DimsLr) {
+        rsum += r;
+        rmax = r > rmax ? r : rmax;
+      }
+      std::fprintf(stderr, "[mxc] level %lld built in %.1f s: %lld boxes, ranks sum %lld max %lld\n", l,
+                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), A[l].nboxes, rsum, rmax);
+      std::fflush(stderr);
+    }
   }
   local_bodies = {0, nbodies};
 }

@@ -289,3 +289,24 @@ def test_batched_right_hand_sides_match_single(setup, operators, preconditioners
         r1, i1, n1 = A.gmres(M, db, dx, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2, inner_precision=N.MXG_FP32)
         assert res[q][-1] < 1e-11 and (its[q], inn[q]) == (i1, n1)
         assert np.array_equal(Xb[:, q], dx.download()), q
+
+
+def test_zero_right_hand_side_in_a_batch(setup, operators, preconditioners):
+    """A zero column among nonzero ones: its solution stays zero and takes no
+    cycle (the reference's loop also ends at once there: |r|/|b| is NaN, which
+    no tolerance test passes), and the other columns equal their single
+    solves bit for bit."""
+    A, _ = operators
+    M, _ = preconditioners
+    ctx = setup["ctx"]
+    M.factorize(N.MXG_FP64)
+    X = np.stack([O.uniform_vec(NPTS, -1, 1, 40), np.zeros(NPTS), O.uniform_vec(NPTS, -1, 1, 41)], axis=1)
+    dB, dX = ctx.array(X), ctx.zeros((NPTS, 3))
+    res, its, inn = A.gmres_batch(M, dB, dX, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2)
+    Xb = dX.download()
+    assert its[1] == 0 and inn[1] == 0 and res[1][-1] == 0.0 and not Xb[:, 1].any()
+    for q in (0, 2):
+        db, dx = ctx.array(X[:, q]), ctx.zeros(NPTS)
+        r1, i1, n1 = A.gmres(M, db, dx, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2)
+        assert res[q][-1] < 1e-11 and (its[q], inn[q]) == (i1, n1)
+        assert np.array_equal(Xb[:, q], dx.download()), q
