@@ -101,6 +101,8 @@ def gpu() -> C.CDLL:
             "mxg_h2_create": [P, MxgKernel, P, c_ll, c_ll, c_int, C.POINTER(P)],
             "mxg_h2_destroy": [P],
             "mxg_h2_define_level": [P, c_ll, c_ll, P, P, P, P, P, P],
+            "mxg_h2_construct_level": [P, c_ll, c_ll, P, P, P, P, P, P, P, P, c_d, c_ll, c_int, P],
+            "mxg_h2_construct_level_runs": [P, c_ll, c_ll, P, P, P, P, P, P, P, P, c_d, c_ll, c_int, P],
             "mxg_h2_upload": [P, c_ll, c_int, P, c_ll],
             "mxg_ulv_create": [P, c_int, C.POINTER(P)],
             "mxg_ulv_destroy": [P],
