@@ -10,13 +10,19 @@
 // generated in registers/LDS: the signs from a 64-bit mix of (t, i / 64), the
 // kernel entries from the body coordinates.
 //
-// Work decomposition: one 256-thread workgroup owns a 256 (sketch rows) x 64
-// (columns) tile of Y and a contiguous slice of far rows; slices of one box
-// are written as separate partial tiles and summed in a fixed order, so the
-// result is deterministic. Per step of 16 far rows the workgroup evaluates a
-// 16 x 64 block of G into LDS (4 entries per thread), computes the 16 sign
-// words its four waves need, and each wave runs 4 x 4 f64 MFMA 16x16x4 tiles
-// over its 64 sketch rows (v_mfma_f64_16x16x4f64, 64 accumulators per lane).
+// Work decomposition: one 256-thread workgroup owns a (64 RT sketch rows) x
+// 64 (columns) tile of Y and a contiguous slice of far rows; slices of one
+// box are written as separate partial tiles and summed in a fixed order, so
+// the result is deterministic. RT, the 16-row MFMA tiles per wave, follows
+// the box size (4 up to M = 256, then 5, 6 or 8), so a box of up to 512
+// columns is one tile high and its kernel entries are evaluated once. Per step
+// of 16 far rows the workgroup evaluates a 16 x 64 block of G into LDS (4
+// entries per thread), computes the sign words of its rows, and each wave runs
+// RT x 4 f64 MFMA 16x16x4 tiles over its 16 RT sketch rows
+// (v_mfma_f64_16x16x4f64, 16 RT accumulators of 4 doubles per lane). Every
+// entry of Y sums the far rows in the same order for every RT.
+#include <map>
+
 #include "internal.hpp"
 
 namespace mxg {
@@ -24,7 +30,7 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int SK_TI = 256, SK_TJ = 64, SK_BK = 16;
+constexpr int SK_TJ = 64, SK_BK = 16;
 
 __device__ inline unsigned long long sketch_word(unsigned long long t, unsigned long long b) {
   unsigned long long z = t * 0x9E3779B97F4A7C15ull + b * 0xD1B54A32D192ED03ull + 0x632BE59BD9B4E019ull;
@@ -33,10 +39,13 @@ __device__ inline unsigned long long sketch_word(unsigned long long t, unsigned 
   return z ^ (z >> 31);
 }
 
+template <int RT>
 __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ tasks, const double* __restrict__ bodies,
                                                 int kid, double diag, double alpha) {
+  constexpr int WR = 16 * RT;  // sketch rows per wave
+  constexpr int NG = RT;       // 64-row groups of the tile (4 WR / 64)
   __shared__ double Gs[SK_BK][SK_TJ + 1];
-  __shared__ unsigned long long Ws[SK_BK][4];
+  __shared__ unsigned long long Ws[SK_BK][NG];
   __shared__ double Xs[SK_TJ][3];
   __shared__ double Xr[SK_BK][3];  // coordinates of the step's far rows
   const SketchTask tk = tasks[blockIdx.x];
@@ -70,14 +79,15 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
     idx_next = t1 < tk.t1 ? tk.rows[t1] : -1;
   }
 
-  d4 acc[4][4];
+  d4 acc[RT][4];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a = 0; a < RT; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = d4{0., 0., 0., 0.};
 
-  const int irow0 = tk.i0 + wave * 64;
+  const int irow0 = tk.i0 + wave * WR;
   const bool active = irow0 < tk.M;
+  const int ntile = min(RT, (tk.M - irow0 + 15) / 16);  // row tiles of this wave inside the box
   const int jj = tid & 63;
   const bool jvalid = tk.j0 + jj < tk.M;
 
@@ -98,25 +108,28 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
       if (t < tk.t1 && jvalid) v = kernel_eval(kid, diag, alpha, dist2(Xr[tt], Xs[jj]));
       Gs[tt][jj] = v;
     }
-    if (tid < 4 * SK_BK) {
-      const int tt = tid >> 2, b = tid & 3;
+    if (tid < NG * SK_BK) {
+      const int tt = tid / NG, b = tid - NG * (tid / NG);
       Ws[tt][b] = sketch_word((unsigned long long)(tb + tt), (unsigned long long)((tk.i0 >> 6) + b));
     }
     __syncthreads();
     if (active) {
 #pragma unroll
       for (int kk = 0; kk < SK_BK; kk += 4) {
-        const unsigned long long w = Ws[kk + lk][wave];
-        double a[4], g[4];
-#pragma unroll
-        for (int bi = 0; bi < 4; bi++) a[bi] = ((w >> (bi * 16 + lr)) & 1ull) ? -1. : 1.;
+        double g[4];
 #pragma unroll
         for (int bj = 0; bj < 4; bj++) g[bj] = Gs[kk + lk][bj * 16 + lr];
 #pragma unroll
-        for (int bi = 0; bi < 4; bi++)
+        for (int bi = 0; bi < RT; bi++) {
+          if (bi < ntile) {  // wave-uniform
+            // sketch row (within the tile) wave WR + 16 bi + lr: bit (row mod 64) of word row / 64
+            const int row = wave * WR + bi * 16 + lr;
+            const double a = ((Ws[kk + lk][row >> 6] >> (row & 63)) & 1ull) ? -1. : 1.;
 #pragma unroll
-          for (int bj = 0; bj < 4; bj++)
-            acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], g[bj], acc[bi][bj], 0, 0, 0);
+            for (int bj = 0; bj < 4; bj++)
+              acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, g[bj], acc[bi][bj], 0, 0, 0);
+          }
+        }
       }
     }
     // every read of Xr for this step came before the barrier above
@@ -128,7 +141,7 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
 
   if (!active) return;
 #pragma unroll
-  for (int bi = 0; bi < 4; bi++)
+  for (int bi = 0; bi < RT; bi++)
 #pragma unroll
     for (int bj = 0; bj < 4; bj++)
 #pragma unroll
@@ -152,11 +165,14 @@ __global__ void __launch_bounds__(256) k_sum_slices(const SliceSum* __restrict__
   tk.dst[e] = s;
 }
 
+// row tiles of 16 per wave for a box of M columns: one tile high up to 512
+int sketch_row_tiles(int M) { return M <= 256 ? 4 : M <= 320 ? 5 : M <= 384 ? 6 : 8; }
+
 }  // namespace
 
 void sketch_rows(mxg_ctx_s* ctx, const std::vector<SketchBox>& boxes, const double* d_bodies, mxg_kernel_t k,
                  double* d_partial, hipStream_t s) {
-  std::vector<SketchTask> tasks;
+  std::map<int, std::vector<SketchTask>> tasks;  // by row tiles per wave
   std::vector<SliceSum> sums;
   std::vector<long long> prefix{0};
   size_t poff = 0;
@@ -167,11 +183,12 @@ void sketch_rows(mxg_ctx_s* ctx, const std::vector<SketchBox>& boxes, const doub
     const int nsplit = (int)((b.nf + SKETCH_ROWS - 1) / SKETCH_ROWS);
     const long long len = (long long)b.M * b.M;
     double* base = nsplit == 1 ? b.Y : d_partial + poff;
+    const int rt = sketch_row_tiles(b.M), ti = 64 * rt;
     for (int sp = 0; sp < nsplit; sp++) {
       const int t0 = sp * SKETCH_ROWS, t1 = (int)std::min<long long>(b.nf, (long long)(sp + 1) * SKETCH_ROWS);
-      for (int i0 = 0; i0 < b.M; i0 += SK_TI)
+      for (int i0 = 0; i0 < b.M; i0 += ti)
         for (int j0 = 0; j0 < b.M; j0 += SK_TJ)
-          tasks.push_back(SketchTask{b.rows, b.cols, base + sp * len, t0, t1, b.M, i0, j0});
+          tasks[rt].push_back(SketchTask{b.rows, b.cols, base + sp * len, t0, t1, b.M, i0, j0});
     }
     if (nsplit > 1) {
       sums.push_back(SliceSum{base, b.Y, len, nsplit});
@@ -180,12 +197,25 @@ void sketch_rows(mxg_ctx_s* ctx, const std::vector<SketchBox>& boxes, const doub
     }
   }
   if (tasks.empty()) return;
-  MXG_REQUIRE(tasks.size() < (1u << 31), "sketch grid too large");
-  DevBuf<SketchTask> dt;
-  dt.upload(tasks, s);
+  std::vector<DevBuf<SketchTask>> dts(tasks.size());
+  size_t q = 0;
+  for (const auto& kv : tasks) {
+    MXG_REQUIRE(kv.second.size() < (1u << 31), "sketch grid too large");
+    dts[q++].upload(kv.second, s);
+  }
   ProfScope prof(ctx, MXG_PROF_SKETCH, s, flops);
-  hipLaunchKernelGGL(k_sketch, dim3((unsigned)tasks.size()), dim3(256), 0, s, dt.p, d_bodies, (int)k.id, k.diag, k.alpha);
-  MXG_HIP(hipGetLastError());
+  q = 0;
+  for (const auto& kv : tasks) {
+    const dim3 g((unsigned)kv.second.size()), blk(256);
+    const SketchTask* tp = dts[q++].p;
+    switch (kv.first) {
+      case 4: hipLaunchKernelGGL(k_sketch<4>, g, blk, 0, s, tp, d_bodies, (int)k.id, k.diag, k.alpha); break;
+      case 5: hipLaunchKernelGGL(k_sketch<5>, g, blk, 0, s, tp, d_bodies, (int)k.id, k.diag, k.alpha); break;
+      case 6: hipLaunchKernelGGL(k_sketch<6>, g, blk, 0, s, tp, d_bodies, (int)k.id, k.diag, k.alpha); break;
+      default: hipLaunchKernelGGL(k_sketch<8>, g, blk, 0, s, tp, d_bodies, (int)k.id, k.diag, k.alpha); break;
+    }
+    MXG_HIP(hipGetLastError());
+  }
   DevBuf<SliceSum> ds;
   DevBuf<long long> dp;
   if (!sums.empty()) {
