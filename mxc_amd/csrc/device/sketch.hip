@@ -88,6 +88,7 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
   const int irow0 = tk.i0 + wave * WR;
   const bool active = irow0 < tk.M;
   const int ntile = min(RT, (tk.M - irow0 + 15) / 16);  // row tiles of this wave inside the box
+  const int ctile = min(4, (tk.M - tk.j0 + 15) / 16);    // column tiles of the tile inside the box
   const int jj = tid & 63;
   const bool jvalid = tk.j0 + jj < tk.M;
 
@@ -127,7 +128,8 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
             const double a = ((Ws[kk + lk][row >> 6] >> (row & 63)) & 1ull) ? -1. : 1.;
 #pragma unroll
             for (int bj = 0; bj < 4; bj++)
-              acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, g[bj], acc[bi][bj], 0, 0, 0);
+              if (bj < ctile)  // wave-uniform: column tiles past the box hold zeros
+                acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, g[bj], acc[bi][bj], 0, 0, 0);
           }
         }
       }
