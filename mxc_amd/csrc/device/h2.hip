@@ -204,8 +204,10 @@ template void convert<double, double>(double*, const double*, size_t, hipStream_
 
 namespace {
 
-// Budget of doubles for the far-sample matrices G of one construction chunk.
-constexpr size_t G_BUDGET = size_t(1) << 28;
+// Budget of doubles (8 GiB) for the far-sample matrices G and the sketch's
+// slice partials of one construction chunk: the boxes of a chunk share one
+// sketch launch per tile height, so larger chunks mean fewer, fuller launches.
+constexpr size_t G_BUDGET = size_t(1) << 30;
 
 // The far-body rows of a level's boxes on the device: box i's rows are
 // d[off[i] .. off[i] + nf[i]).

@@ -117,19 +117,23 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
     if (active) {
 #pragma unroll
       for (int kk = 0; kk < SK_BK; kk += 4) {
-        double g[4];
+        double g[4], a[RT];
 #pragma unroll
         for (int bj = 0; bj < 4; bj++) g[bj] = Gs[kk + lk][bj * 16 + lr];
+        // sketch row (within the tile) x + lr, x = wave WR + 16 bi: bit (row mod
+        // 64) of word row / 64; x is a multiple of 16, so row / 64 = x / 64
+#pragma unroll
+        for (int bi = 0; bi < RT; bi++) {
+          const int x = wave * WR + bi * 16;
+          a[bi] = ((Ws[kk + lk][x >> 6] >> ((x & 63) + lr)) & 1ull) ? -1. : 1.;
+        }
 #pragma unroll
         for (int bi = 0; bi < RT; bi++) {
           if (bi < ntile) {  // wave-uniform
-            // sketch row (within the tile) wave WR + 16 bi + lr: bit (row mod 64) of word row / 64
-            const int row = wave * WR + bi * 16 + lr;
-            const double a = ((Ws[kk + lk][row >> 6] >> (row & 63)) & 1ull) ? -1. : 1.;
 #pragma unroll
             for (int bj = 0; bj < 4; bj++)
               if (bj < ctile)  // wave-uniform: column tiles past the box hold zeros
-                acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, g[bj], acc[bi][bj], 0, 0, 0);
+                acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], g[bj], acc[bi][bj], 0, 0, 0);
           }
         }
       }
