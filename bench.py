@@ -49,8 +49,9 @@ def parse():
     ap.add_argument("--sampling", type=int, default=2000)
     ap.add_argument("--inner-eps", type=float, default=1e-7,
                     help="tolerance of the operator the mixed path applies in its FP32 inner iterations (0: the main operator)")
-    ap.add_argument("--mixed-factor", choices=["bf16", "fp32"], default="bf16",
-                    help="ULV of the mixed path: FP32 factorization with bfloat16-stored or FP32-stored factors")
+    ap.add_argument("--mixed-factor", choices=["bf16", "fp32", "bf16schur"], default="bf16",
+                    help="ULV of the mixed path: FP32 factorization with bfloat16-stored or FP32-stored factors "
+                         "(bf16schur: bfloat16-stored, and the Schur-complement products on bfloat16 inputs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=16384, help="points of the CPU baseline sample")
     ap.add_argument("--workload", choices=["solve", "batch", "serve"], default="solve",
@@ -212,7 +213,8 @@ def main():
 
     fflops = factor_flops(M, pts, L, theta)
     fp64 = run_path(N.MXG_FP64, N.MXG_FP64, sol["fp64"])
-    mixed = run_path(N.MXG_BF16 if args.mixed_factor == "bf16" else N.MXG_FP32, N.MXG_FP32, sol["mixed"])
+    mixed_code = {"bf16": N.MXG_BF16, "fp32": N.MXG_FP32, "bf16schur": N.MXG_BF16_SCHUR}[args.mixed_factor]
+    mixed = run_path(mixed_code, N.MXG_FP32, sol["mixed"])
     # error of every mixed-path solution against the FP64 path's for the same right-hand side
     mix_err = []
     for k in range(args.steps):
@@ -255,7 +257,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f32 ULV factorization with bf16-stored factors" if args.mixed_factor == "bf16" else "f32 ULV") +
+        "dtype": {"bf16": "f32 ULV factorization with bf16-stored factors", "fp32": "f32 ULV",
+                  "bf16schur": "f32 ULV factorization with bf16 Schur-complement inputs and bf16-stored factors"}
+                 [args.mixed_factor] +
                  " + f32 inner operator, f64 Arnoldi/residual",
         "data": "synthetic (seeded points and right-hand sides)",
         "config": {

@@ -39,8 +39,12 @@ typedef struct {
 
 /* precision of a factorization or of an operator application; MXG_BF16 (ULV
  * only): factorization in FP32, factors stored in bfloat16 for the
- * substitution, which accumulates in FP32 */
-enum { MXG_FP64 = 0, MXG_FP32 = 1, MXG_BF16 = 2 };
+ * substitution, which accumulates in FP32; MXG_BF16_SCHUR (ULV only): the same,
+ * with the Schur-complement products of the factorization
+ * (h2matrix.cpp:405-424) taking bfloat16 inputs on the bfloat16 matrix cores
+ * (FP32 accumulation). mxg_gemm takes MXG_BF16 for FP32 matrices multiplied
+ * with bfloat16-rounded inputs. */
+enum { MXG_FP64 = 0, MXG_FP32 = 1, MXG_BF16 = 2, MXG_BF16_SCHUR = 3 };
 
 typedef struct mxg_ctx_s* mxg_ctx_t; /* device + stream (replaces deviceHandle_t) */
 typedef struct mxg_h2_s* mxg_h2_t;   /* one H^2 matrix resident on the device   */

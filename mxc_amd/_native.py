@@ -21,6 +21,7 @@ MXG_ERR_ARG = 1  # invalid argument or size beyond a documented limit
 MXG_FP64 = 0
 MXG_FP32 = 1
 MXG_BF16 = 2  # ULV only: FP32 factorization, bfloat16-stored factors for the substitution
+MXG_BF16_SCHUR = 3  # ULV only: as MXG_BF16, with bfloat16 inputs to the Schur-complement products
 KERNELS = {"laplace": 0, "yukawa": 1, "gaussian": 2}
 GMRES_REFERENCE = 0
 GMRES_CGS2 = 1

@@ -121,10 +121,109 @@ __global__ void __launch_bounds__(256) k_bgemm(const GemmTask<T>* __restrict__ t
       }
 }
 
+// The FP32 tasks with bfloat16 inputs: op(A) and op(B) are rounded to
+// bfloat16 (nearest even) as they are staged, and each 16-deep slab is one
+// v_mfma_f32_16x16x16_bf16 per 16 x 16 accumulator (FP32 accumulation). Lane l
+// supplies A[l % 16][4 (l / 16) + e] and B[4 (l / 16) + e][l % 16], e < 4; the
+// accumulator layout is that of the FP32 instruction.
+typedef short s4_t __attribute__((ext_vector_type(4)));
+constexpr int PADH = 2;
+
+__device__ inline unsigned short bf16_bits(float x) {
+  const unsigned u = __float_as_uint(x);
+  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+__global__ void __launch_bounds__(256) k_bgemm_bf16(const GemmTask<float>* __restrict__ tasks,
+                                                    const GemmSeg<float>* __restrict__ segs,
+                                                    const long long* __restrict__ prefix, int ntasks) {
+  __shared__ unsigned short As[BK][BM + PADH];
+  __shared__ unsigned short Bs[BK][BN + PADH];
+  const long long tile = blockIdx.x;
+  const int t = find_task(prefix, ntasks, tile);
+  const GemmTask<float> task = tasks[t];
+  const long long local = tile - prefix[t];
+  const int tiles_m = (task.m + BM - 1) / BM;
+  const int m0 = (int)(local % tiles_m) * BM;
+  const int n0 = (int)(local / tiles_m) * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  f4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = f4{0, 0, 0, 0};
+
+  for (int s = task.seg0; s < task.seg0 + task.nseg; s++) {
+    const GemmSeg<float> g = segs[s];
+    for (int k0 = 0; k0 < g.k; k0 += BK) {
+#pragma unroll
+      for (int e0 = 0; e0 < BM * BK; e0 += 256) {
+        const int e = e0 + tid;
+        int i, p;
+        if (!g.ta) { i = e % BM; p = e / BM; } else { p = e % BK; i = e / BK; }
+        const int gi = m0 + i, gp = k0 + p;
+        float v = 0.f;
+        if (gi < task.m && gp < g.k) {
+          const int sr = g.ta ? gp : gi, sc = g.ta ? gi : gp;
+          if (!(g.amask && sr > sc)) v = g.A[sr + (long long)sc * g.lda];
+        }
+        As[p][i] = bf16_bits(v);
+      }
+#pragma unroll
+      for (int e0 = 0; e0 < BK * BN; e0 += 256) {
+        const int e = e0 + tid;
+        int p, j;
+        if (!g.tb) { p = e % BK; j = e / BK; } else { j = e % BN; p = e / BN; }
+        const int gp = k0 + p, gj = n0 + j;
+        float v = 0.f;
+        if (gp < g.k && gj < task.n) {
+          const int sr = g.tb ? gj : gp, sc = g.tb ? gp : gj;
+          if (!(g.bmask && sr > sc)) v = g.B[sr + (long long)sc * g.ldb];
+        }
+        Bs[p][j] = bf16_bits(v);
+      }
+      __syncthreads();
+      s4_t a[2], b[2];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+#pragma unroll
+        for (int i = 0; i < 2; i++) a[i][e] = (short)As[4 * lk + e][wr * 32 + i * 16 + lr];
+#pragma unroll
+        for (int j = 0; j < 2; j++) b[j][e] = (short)Bs[4 * lk + e][wc * 32 + j * 16 + lr];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+      __syncthreads();
+    }
+  }
+
+  const float alpha = (float)task.alpha, beta = (float)task.beta;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int row = m0 + wr * 32 + i * 16 + 4 * lk + q;
+        const int col = n0 + wc * 32 + j * 16 + lr;
+        if (row < task.m && col < task.n) {
+          float* c = task.C + row + (long long)col * task.ldc;
+          float v = alpha * acc[i][j][q];
+          if (task.beta != 0.) v += beta * (*c);
+          *c = v;
+        }
+      }
+}
+
 }  // namespace
 
 template <class T>
-void gemm_batched(const std::vector<GemmTask<T>>& tasks, const std::vector<GemmSeg<T>>& segs, hipStream_t s) {
+void gemm_batched(const std::vector<GemmTask<T>>& tasks, const std::vector<GemmSeg<T>>& segs, hipStream_t s, bool bf16) {
   if (tasks.empty()) return;
   std::vector<long long> prefix(tasks.size() + 1, 0);
   for (size_t t = 0; t < tasks.size(); t++) {
@@ -142,13 +241,22 @@ void gemm_batched(const std::vector<GemmTask<T>>& tasks, const std::vector<GemmS
   dt.upload(tasks, s);
   ds.upload(segs, s);
   dp.upload(prefix, s);
-  hipLaunchKernelGGL(k_bgemm<T>, dim3((unsigned)total), dim3(256), 0, s, dt.p, ds.p, dp.p, (int)tasks.size());
+  if constexpr (std::is_same<T, float>::value) {
+    if (bf16)
+      hipLaunchKernelGGL(k_bgemm_bf16, dim3((unsigned)total), dim3(256), 0, s, dt.p, ds.p, dp.p, (int)tasks.size());
+    else
+      hipLaunchKernelGGL(k_bgemm<T>, dim3((unsigned)total), dim3(256), 0, s, dt.p, ds.p, dp.p, (int)tasks.size());
+  } else {
+    MXG_REQUIRE(!bf16, "bfloat16 inputs are for FP32 products");
+    hipLaunchKernelGGL(k_bgemm<T>, dim3((unsigned)total), dim3(256), 0, s, dt.p, ds.p, dp.p, (int)tasks.size());
+  }
   MXG_HIP(hipGetLastError());
   MXG_HIP(hipStreamSynchronize(s));  // descriptor arrays are freed on return
 }
 
-template void gemm_batched<double>(const std::vector<GemmTask<double>>&, const std::vector<GemmSeg<double>>&, hipStream_t);
-template void gemm_batched<float>(const std::vector<GemmTask<float>>&, const std::vector<GemmSeg<float>>&, hipStream_t);
+template void gemm_batched<double>(const std::vector<GemmTask<double>>&, const std::vector<GemmSeg<double>>&, hipStream_t,
+                                   bool);
+template void gemm_batched<float>(const std::vector<GemmTask<float>>&, const std::vector<GemmSeg<float>>&, hipStream_t, bool);
 
 }  // namespace mxg
 
@@ -157,11 +265,11 @@ using namespace mxg;
 namespace {
 template <class T>
 void one_gemm(mxg_ctx_t ctx, int ta, int tb, long long m, long long n, long long k, double alpha, const void* A, long long lda,
-              const void* B, long long ldb, double beta, void* C, long long ldc) {
+              const void* B, long long ldb, double beta, void* C, long long ldc, bool bf16 = false) {
   std::vector<GemmSeg<T>> segs{GemmSeg<T>{static_cast<const T*>(A), static_cast<const T*>(B), (int)lda, (int)ldb, (int)k,
                                           (unsigned char)(ta != 0), (unsigned char)(tb != 0), 0, 0}};
   std::vector<GemmTask<T>> tasks{GemmTask<T>{static_cast<T*>(C), (int)ldc, (int)m, (int)n, 0, k > 0 ? 1 : 0, alpha, beta}};
-  gemm_batched<T>(tasks, segs, ctx->stream);
+  gemm_batched<T>(tasks, segs, ctx->stream, bf16);
 }
 }  // namespace
 
@@ -180,6 +288,8 @@ extern "C" int mxg_gemm(mxg_ctx_t ctx, int precision, int ta, int tb, long long 
       one_gemm<double>(ctx, ta, tb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
     else if (precision == MXG_FP32)
       one_gemm<float>(ctx, ta, tb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc);
+    else if (precision == MXG_BF16)
+      one_gemm<float>(ctx, ta, tb, m, n, k, alpha, d_A, lda, d_B, ldb, beta, d_C, ldc, true);
     else
       throw Error(MXG_ERR_ARG, "unknown precision");
   });

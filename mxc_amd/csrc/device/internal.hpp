@@ -39,8 +39,11 @@ struct GemmTask {
   int seg0, nseg;
   double alpha, beta;
 };
+// bf16: FP32 tasks whose inputs are rounded to bfloat16 and multiplied on the
+// bfloat16 matrix cores with FP32 accumulation
 template <class T>
-void gemm_batched(const std::vector<GemmTask<T>>& tasks, const std::vector<GemmSeg<T>>& segs, hipStream_t s);
+void gemm_batched(const std::vector<GemmTask<T>>& tasks, const std::vector<GemmSeg<T>>& segs, hipStream_t s,
+                  bool bf16 = false);
 
 // ---- dense factorizations, one workgroup per matrix (lapack.hip) ----
 // Householder QR of an m x n column-major matrix in place (R on/above the

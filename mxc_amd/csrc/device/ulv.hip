@@ -62,7 +62,8 @@ struct GemmList {
     }
     g.insert(g.end(), o.g.begin(), o.g.end());
   }
-  void run(hipStream_t s) const { gemm_batched<T>(t, g, s); }
+  bool bf16 = false;  // FP32 products with bfloat16 inputs
+  void run(hipStream_t s) const { gemm_batched<T>(t, g, s, bf16); }
 };
 
 // The launches of one level's factorization in their order. Plans of the
@@ -103,6 +104,8 @@ struct LevelPlan {
     }
     sch1.append(o.sch1);
     sch2.append(o.sch2);
+    sch1.bf16 = sch1.bf16 || o.sch1.bf16;
+    sch2.bf16 = sch2.bf16 || o.sch2.bf16;
     for (auto& b : o.scratch) scratch.push_back(std::move(b));
   }
   void run(hipStream_t s) const {
@@ -167,6 +170,7 @@ template <class T, class S = T>
 struct UlvImpl : UlvBase {
   static constexpr bool kCopy = !std::is_same<T, S>::value;
   mxg_h2_s* h;
+  bool schur_bf16 = false;  // Schur-complement products with bfloat16 inputs
   struct Lv {
     DevBuf<T> A, U, V, RRinv, X, Z;
     DevBuf<S> As, Us, Vs;  // the factors in storage precision (kCopy only)
@@ -186,7 +190,7 @@ struct UlvImpl : UlvBase {
   DevBuf<T> ubuf;       // the leaf's backward vectors [w_i ; x_i[s:] - sum t], n per right-hand side
   long long tlen = 1;
 
-  explicit UlvImpl(mxg_h2_s* hh) : h(hh) {
+  explicit UlvImpl(mxg_h2_s* hh, bool sbf16 = false) : h(hh), schur_bf16(sbf16) {
     hipStream_t s = h->ctx->stream;
     lv.resize(h->L + 1);
     for (long long l = 0; l <= h->L; l++) {
@@ -315,6 +319,7 @@ struct UlvImpl : UlvBase {
   void plan_level(long long l, LevelPlan<T>& P) {
     Level& H = h->lv[l];
     Lv& F = lv[l];
+    P.sch1.bf16 = P.sch2.bf16 = schur_bf16;
     const long long nb = H.nb;
     auto M_ = [&](long long i) { return (int)H.dims[i]; };
     auto S_ = [&](long long i) { return (int)H.ranks[i]; };
@@ -559,7 +564,8 @@ int mxg_ulv_create(mxg_h2_t h, int precision, mxg_ulv_t* out) {
     MXG_REQUIRE(h && out, "null argument");
     for (long long l = 0; l <= h->L; l++) MXG_REQUIRE(h->lv[l].built, "H2 matrix not fully constructed");
     MXG_REQUIRE(h->flags & MXG_H2_FACTORABLE, "H2 matrix was created without MXG_H2_FACTORABLE");
-    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32 || precision == MXG_BF16, "unknown precision");
+    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32 || precision == MXG_BF16 || precision == MXG_BF16_SCHUR,
+                "unknown precision");
     MXG_HIP(hipSetDevice(h->ctx->device));
     auto f = std::make_unique<mxg_ulv_s>();
     f->h = h;
@@ -569,7 +575,7 @@ int mxg_ulv_create(mxg_h2_t h, int precision, mxg_ulv_t* out) {
     else if (precision == MXG_FP32)
       f->impl = std::make_unique<UlvImpl<float>>(h);
     else
-      f->impl = std::make_unique<UlvImpl<float, bf16_t>>(h);
+      f->impl = std::make_unique<UlvImpl<float, bf16_t>>(h, precision == MXG_BF16_SCHUR);
     // device time of the factorization proper (every level's launches and the
     // host work that issues them), bracketed by events on the context stream
     hipEvent_t e0, e1;
@@ -591,7 +597,8 @@ int mxg_ulv_create(mxg_h2_t h, int precision, mxg_ulv_t* out) {
 int mxg_ulv_create_batch(const mxg_h2_t* hs, long long count, int precision, mxg_ulv_t* out) {
   return guarded([&] {
     MXG_REQUIRE(hs && out && count >= 1, "null argument or empty batch");
-    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32 || precision == MXG_BF16, "unknown precision");
+    MXG_REQUIRE(precision == MXG_FP64 || precision == MXG_FP32 || precision == MXG_BF16 || precision == MXG_BF16_SCHUR,
+                "unknown precision");
     mxg_ctx_t ctx = hs[0] ? hs[0]->ctx : nullptr;
     for (long long p = 0; p < count; p++) {
       MXG_REQUIRE(hs[p], "null matrix in the batch");
@@ -609,7 +616,7 @@ int mxg_ulv_create_batch(const mxg_h2_t* hs, long long count, int precision, mxg
         fs[p] = std::make_unique<mxg_ulv_s>();
         fs[p]->h = hs[p];
         fs[p]->precision = precision;
-        auto impl = std::make_unique<Impl>(hs[p]);
+        auto impl = std::make_unique<Impl>(hs[p], precision == MXG_BF16_SCHUR);
         impls.push_back(impl.get());
         fs[p]->impl = std::move(impl);
       }

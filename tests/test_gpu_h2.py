@@ -230,6 +230,42 @@ def test_gmres_bf16_preconditioner(setup, operators, preconditioners):
         M.factorize(N.MXG_FP64)
 
 
+def test_bf16_schur_products(setup, operators, preconditioners):
+    """MXG_BF16_SCHUR: the Schur-complement products of the factorization on
+    bfloat16 inputs. Its substitution differs from the FP32 one with
+    bfloat16-stored factors (so the products did change) but stays within a
+    few percent of the FP64 substitution, and GMRES-IR with it reaches the
+    tolerance and the FP64 solution."""
+    A, _ = operators
+    M, _ = preconditioners
+    ctx = setup["ctx"]
+    b = O.uniform_vec(NPTS, -1, 1, 19)
+    zs = {}
+    try:
+        for prec in (N.MXG_FP64, N.MXG_BF16, N.MXG_BF16_SCHUR):
+            M.factorize(prec)
+            d = ctx.array(b)
+            M.precondition(d)
+            zs[prec] = d.download()
+        e16 = O.rel_err(zs[N.MXG_BF16], zs[N.MXG_FP64])
+        e_schur = O.rel_err(zs[N.MXG_BF16_SCHUR], zs[N.MXG_FP64])
+        print(f"substitution vs FP64: bf16-stored {e16:.3e}, bf16 Schur products {e_schur:.3e}")
+        assert not np.array_equal(zs[N.MXG_BF16_SCHUR], zs[N.MXG_BF16])
+        assert e_schur < 5e-2
+        db = ctx.array(b)
+        dx64 = ctx.zeros(NPTS)
+        M.factorize(N.MXG_FP64)
+        A.gmres(M, db, dx64, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2)
+        M.factorize(N.MXG_BF16_SCHUR)
+        dx = ctx.zeros(NPTS)
+        resid, iters, inner = A.gmres(M, db, dx, 1e-11, restart=30, max_cycles=10, mode=N.GMRES_CGS2,
+                                      inner_precision=N.MXG_FP32)
+        assert resid[-1] < 1e-11, resid
+        assert O.rel_err(dx.download(), dx64.download()) < 1e-9
+    finally:
+        M.factorize(N.MXG_FP64)
+
+
 def test_gmres_with_looser_inner_operator(setup, operators, preconditioners):
     """Iterative refinement: FP32 inner iterations on an operator built at
     1e-6, every cycle's residual on the accurate operator, reaches the same

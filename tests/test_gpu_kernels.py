@@ -107,3 +107,36 @@ def test_gemm_beta_zero_ignores_nan(ctx):
     dA, dC = ctx.array(A), ctx.array(Cm)
     N.check_gpu(N.gpu().mxg_gemm(ctx.handle, 0, 0, 0, m, n, k, 1.0, dA.ptr, m, dA.ptr, m, 0.0, dC.ptr, m))
     assert np.array_equal(dC.download(), np.eye(m))
+
+
+def bf16_round(a):
+    """float32 values rounded to bfloat16 (nearest even), as float64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32)
+    return r.view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (70, 45, 33), (130, 129, 65), (64, 64, 16), (5, 200, 300)])
+def test_gemm_bf16_inputs_match_numpy(ctx, ta, tb, m, n, k):
+    """FP32 matrices multiplied on the bfloat16 matrix cores: equal to the
+    FP64 product of the bfloat16-rounded inputs up to FP32 accumulation."""
+    rng = np.random.default_rng(m * 1000 + n * 10 + k + 7 * ta + 3 * tb + 1)
+    A = rng.standard_normal((k, m) if ta else (m, k)).astype(np.float32)
+    B = rng.standard_normal((n, k) if tb else (k, n)).astype(np.float32)
+    Cm = rng.standard_normal((m, n)).astype(np.float32)
+    alpha, beta = 0.75, -1.25
+    dA, dB, dC = ctx.array(np.asfortranarray(A)), ctx.array(np.asfortranarray(B)), ctx.array(np.asfortranarray(Cm))
+    N.check_gpu(N.gpu().mxg_gemm(ctx.handle, N.MXG_BF16, ta, tb, m, n, k, alpha, dA.ptr, A.shape[0], dB.ptr,
+                                 B.shape[0], beta, dC.ptr, m))
+    got = dC.download().astype(np.float64)
+    opA = bf16_round(A.T if ta else A)
+    opB = bf16_round(B.T if tb else B)
+    ref = alpha * (opA @ opB) + beta * Cm.astype(np.float64)
+    scale = np.abs(alpha) * (np.abs(opA) @ np.abs(opB)) + abs(beta) * np.abs(Cm)
+    assert np.max(np.abs(got - ref) / scale) < 2e-5
+    # and it is not the FP32 product: the inputs were rounded
+    if k >= 16:
+        full = alpha * (np.asarray(A.T if ta else A, np.float64) @ np.asarray(B.T if tb else B, np.float64)) \
+            + beta * Cm.astype(np.float64)
+        assert np.max(np.abs(got - full) / scale) > 1e-4
