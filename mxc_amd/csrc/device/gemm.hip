@@ -58,36 +58,77 @@ __global__ void __launch_bounds__(256) k_bgemm(const GemmTask<T>* __restrict__ t
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = V{0, 0, 0, 0};
 
-  for (int s = task.seg0; s < task.seg0 + task.nseg; s++) {
-    const GemmSeg<T> g = segs[s];
-    for (int k0 = 0; k0 < g.k; k0 += BK) {
+  // Slabs of BK columns of op(A) / rows of op(B) over the task's segments in
+  // order; the next slab's global loads are issued before the current one's
+  // MFMAs, so their latency overlaps the matrix cores. The products run over
+  // k in the same order as without the prefetch.
+  constexpr int LA = BM * BK / 256, LB = BK * BN / 256;
+  T ra[LA], rb[LB];
+  int cs = task.seg0, ck = 0;  // the slab to load next
+  const int send = task.seg0 + task.nseg;
+  auto skip_empty = [&]() {
+    while (cs < send && ck >= segs[cs].k) {
+      cs++;
+      ck = 0;
+    }
+  };
+  auto load = [&]() {
+    const GemmSeg<T> g = segs[cs];
 #pragma unroll
-      for (int e0 = 0; e0 < BM * BK; e0 += 256) {
-        const int e = e0 + tid;
-        int i, p;
-        if (!g.ta) { i = e % BM; p = e / BM; } else { p = e % BK; i = e / BK; }
-        const int gi = m0 + i, gp = k0 + p;
-        T v = T(0);
-        if (gi < task.m && gp < g.k) {
-          const int sr = g.ta ? gp : gi, sc = g.ta ? gi : gp;
-          if (!(g.amask && sr > sc)) v = g.A[sr + (long long)sc * g.lda];
-        }
-        As[p][i] = v;
+    for (int u = 0; u < LA; u++) {
+      const int e = u * 256 + tid;
+      int i, p;
+      if (!g.ta) { i = e % BM; p = e / BM; } else { p = e % BK; i = e / BK; }
+      const int gi = m0 + i, gp = ck + p;
+      T v = T(0);
+      if (gi < task.m && gp < g.k) {
+        const int sr = g.ta ? gp : gi, sc = g.ta ? gi : gp;
+        if (!(g.amask && sr > sc)) v = g.A[sr + (long long)sc * g.lda];
       }
+      ra[u] = v;
+    }
 #pragma unroll
-      for (int e0 = 0; e0 < BK * BN; e0 += 256) {
-        const int e = e0 + tid;
-        int p, j;
-        if (!g.tb) { p = e % BK; j = e / BK; } else { j = e % BN; p = e / BN; }
-        const int gp = k0 + p, gj = n0 + j;
-        T v = T(0);
-        if (gp < g.k && gj < task.n) {
-          const int sr = g.tb ? gj : gp, sc = g.tb ? gp : gj;
-          if (!(g.bmask && sr > sc)) v = g.B[sr + (long long)sc * g.ldb];
-        }
-        Bs[p][j] = v;
+    for (int u = 0; u < LB; u++) {
+      const int e = u * 256 + tid;
+      int p, j;
+      if (!g.tb) { p = e % BK; j = e / BK; } else { j = e % BN; p = e / BN; }
+      const int gp = ck + p, gj = n0 + j;
+      T v = T(0);
+      if (gp < g.k && gj < task.n) {
+        const int sr = g.tb ? gj : gp, sc = g.tb ? gp : gj;
+        if (!(g.bmask && sr > sc)) v = g.B[sr + (long long)sc * g.ldb];
       }
+      rb[u] = v;
+    }
+    ck += BK;
+  };
+  auto store = [&](const GemmSeg<T>& g) {
+#pragma unroll
+    for (int u = 0; u < LA; u++) {
+      const int e = u * 256 + tid;
+      int i, p;
+      if (!g.ta) { i = e % BM; p = e / BM; } else { p = e % BK; i = e / BK; }
+      As[p][i] = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < LB; u++) {
+      const int e = u * 256 + tid;
+      int p, j;
+      if (!g.tb) { p = e % BK; j = e / BK; } else { j = e % BN; p = e / BN; }
+      Bs[p][j] = rb[u];
+    }
+  };
+  skip_empty();
+  if (cs < send) {
+    int ls = cs;  // segment of the slab held in ra / rb
+    load();
+    for (;;) {
+      store(segs[ls]);
       __syncthreads();
+      skip_empty();
+      const bool more = cs < send;
+      ls = cs;
+      if (more) load();
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         const T a0 = As[kk + lk][wr * 32 + lr];
@@ -100,6 +141,7 @@ __global__ void __launch_bounds__(256) k_bgemm(const GemmTask<T>* __restrict__ t
         acc[1][1] = mfma(a1, b1, acc[1][1]);
       }
       __syncthreads();
+      if (!more) break;
     }
   }
 
