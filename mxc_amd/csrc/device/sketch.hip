@@ -14,8 +14,9 @@
 // 64 (columns) tile of Y and a contiguous slice of far rows; slices of one
 // box are written as separate partial tiles and summed in a fixed order, so
 // the result is deterministic. RT, the 16-row MFMA tiles per wave, follows
-// the box size (4 up to M = 256, then 5, 6 or 8), so a box of up to 512
-// columns is one tile high and its kernel entries are evaluated once. Per step
+// the box size (the fewest 64-row groups that cover M, up to 6, then 8), so
+// a box of up to 512 columns is one tile high and its kernel entries are
+// evaluated once, and below M = 256 no wave of the tile runs short of rows. Per step
 // of 16 far rows the workgroup evaluates a 16 x 64 block of G into LDS (4
 // entries per thread), computes the sign words of its rows, and each wave runs
 // RT x 4 f64 MFMA 16x16x4 tiles over its 16 RT sketch rows
@@ -171,8 +172,9 @@ __global__ void __launch_bounds__(256) k_sum_slices(const SliceSum* __restrict__
   tk.dst[e] = s;
 }
 
-// row tiles of 16 per wave for a box of M columns: one tile high up to 512
-int sketch_row_tiles(int M) { return M <= 256 ? 4 : M <= 320 ? 5 : M <= 384 ? 6 : 8; }
+// row tiles of 16 per wave for a box of M columns: one tile high up to 512,
+// and every wave holds sketch rows of the box while 64 RT >= M
+int sketch_row_tiles(int M) { return M <= 384 ? std::max(1, (M + 63) / 64) : 8; }
 
 }  // namespace
 
@@ -215,6 +217,9 @@ void sketch_rows(mxg_ctx_s* ctx, const std::vector<SketchBox>& boxes, const doub
     const dim3 g((unsigned)kv.second.size()), blk(256);
     const SketchTask* tp = dts[q++].p;
     switch (kv.first) {
+      case 1: hipLaunchKernelGGL(k_sketch<1>, g, blk, 0, s, tp, d_bodies, (int)k.id, k.diag, k.alpha); break;
+      case 2: hipLaunchKernelGGL(k_sketch<2>, g, blk, 0, s, tp, d_bodies, (int)k.id, k.diag, k.alpha); break;
+      case 3: hipLaunchKernelGGL(k_sketch<3>, g, blk, 0, s, tp, d_bodies, (int)k.id, k.diag, k.alpha); break;
       case 4: hipLaunchKernelGGL(k_sketch<4>, g, blk, 0, s, tp, d_bodies, (int)k.id, k.diag, k.alpha); break;
       case 5: hipLaunchKernelGGL(k_sketch<5>, g, blk, 0, s, tp, d_bodies, (int)k.id, k.diag, k.alpha); break;
       case 6: hipLaunchKernelGGL(k_sketch<6>, g, blk, 0, s, tp, d_bodies, (int)k.id, k.diag, k.alpha); break;
