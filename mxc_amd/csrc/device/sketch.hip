@@ -16,12 +16,14 @@
 // the result is deterministic. RT, the 16-row MFMA tiles per wave, follows
 // the box size (the fewest 64-row groups that cover M, up to 6, then 8), so
 // a box of up to 512 columns is one tile high and its kernel entries are
-// evaluated once, and below M = 256 no wave of the tile runs short of rows. Per step
-// of 16 far rows the workgroup evaluates a 16 x 64 block of G into LDS (4
-// entries per thread), computes the sign words of its rows, and each wave runs
-// RT x 4 f64 MFMA 16x16x4 tiles over its 16 RT sketch rows
-// (v_mfma_f64_16x16x4f64, 16 RT accumulators of 4 doubles per lane). Every
-// entry of Y sums the far rows in the same order for every RT.
+// evaluated once, and up to M = 384 the tile is less than 64 rows taller than
+// the box. Per step of 16 far rows the workgroup evaluates a 16 x 64 block of
+// G into LDS (4 entries per thread), computes the sign words of its rows, and
+// each wave with rows in the box runs all RT x 4 f64 MFMA 16x16x4 tiles over
+// its 16 RT sketch rows back to back (v_mfma_f64_16x16x4f64, 16 RT
+// accumulators of 4 doubles per lane), tiles past the box included: their
+// sums are not stored, and no test separates the MFMAs. Every entry of Y sums
+// the far rows in the same order for every RT.
 #include <map>
 
 #include "internal.hpp"
@@ -40,9 +42,13 @@ __device__ inline unsigned long long sketch_word(unsigned long long t, unsigned 
   return z ^ (z >> 31);
 }
 
+// Tiles of up to 192 rows are held to three waves per SIMD (168 registers a
+// lane; RT = 3 then keeps a few values of the entry evaluation in scratch),
+// where the compiler would settle for two, so that more workgroups' entry
+// evaluations overlap other workgroups' MFMAs.
 template <int RT>
-__global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ tasks, const double* __restrict__ bodies,
-                                                int kid, double diag, double alpha) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RT <= 3 ? 3 : 1)))
+k_sketch(const SketchTask* __restrict__ tasks, const double* __restrict__ bodies, int kid, double diag, double alpha) {
   constexpr int WR = 16 * RT;  // sketch rows per wave
   constexpr int NG = RT;       // 64-row groups of the tile (4 WR / 64)
   __shared__ double Gs[SK_BK][SK_TJ + 1];
@@ -50,7 +56,9 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
   __shared__ double Xs[SK_TJ][3];
   __shared__ double Xr[SK_BK][3];  // coordinates of the step's far rows
   const SketchTask tk = tasks[blockIdx.x];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wave index as a wave-uniform value, so that the tile-extent tests
+  // below are scalar branches rather than exec masks around every MFMA
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lk = lane >> 4;
 
   if (tid < SK_TJ) {
@@ -88,8 +96,6 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
 
   const int irow0 = tk.i0 + wave * WR;
   const bool active = irow0 < tk.M;
-  const int ntile = min(RT, (tk.M - irow0 + 15) / 16);  // row tiles of this wave inside the box
-  const int ctile = min(4, (tk.M - tk.j0 + 15) / 16);    // column tiles of the tile inside the box
   const int jj = tid & 63;
   const bool jvalid = tk.j0 + jj < tk.M;
 
@@ -115,6 +121,8 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
       Ws[tt][b] = sketch_word((unsigned long long)(tb + tt), (unsigned long long)((tk.i0 >> 6) + b));
     }
     __syncthreads();
+    // every row and column tile of the wave, also those past the box (their
+    // sums are not stored): the RT x 4 MFMAs of a step run back to back
     if (active) {
 #pragma unroll
       for (int kk = 0; kk < SK_BK; kk += 4) {
@@ -129,14 +137,10 @@ __global__ void __launch_bounds__(256) k_sketch(const SketchTask* __restrict__ t
           a[bi] = ((Ws[kk + lk][x >> 6] >> ((x & 63) + lr)) & 1ull) ? -1. : 1.;
         }
 #pragma unroll
-        for (int bi = 0; bi < RT; bi++) {
-          if (bi < ntile) {  // wave-uniform
+        for (int bi = 0; bi < RT; bi++)
 #pragma unroll
-            for (int bj = 0; bj < 4; bj++)
-              if (bj < ctile)  // wave-uniform: column tiles past the box hold zeros
-                acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], g[bj], acc[bi][bj], 0, 0, 0);
-          }
-        }
+          for (int bj = 0; bj < 4; bj++)
+            acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[bi], g[bj], acc[bi][bj], 0, 0, 0);
       }
     }
     // every read of Xr for this step came before the barrier above
