@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -98,6 +99,71 @@ struct DevBuf {
   }
 };
 
+// Descriptor arrays (task lists) of a sequence of launches on one stream. An
+// array is copied into pinned host memory and from there, by an asynchronous
+// copy on the stream, into device memory; both stay valid until rewind(),
+// which first waits for the stream. Launches that read them need no
+// synchronization of their own, so the host builds a launch's descriptors
+// while the GPU runs the earlier launches. Blocks are kept for later use.
+class Stage {
+ public:
+  Stage() = default;
+  Stage(const Stage&) = delete;
+  Stage& operator=(const Stage&) = delete;
+  ~Stage() {
+    for (Block& b : blocks_) {
+      if (b.d) (void)hipFree(b.d);
+      if (b.h) (void)hipHostFree(b.h);
+    }
+  }
+  void rewind(hipStream_t s) {
+    MXG_HIP(hipStreamSynchronize(s));
+    cur_ = 0;
+    off_ = 0;
+  }
+  // device copy of v (nullptr for an empty v), valid until rewind()
+  template <class X>
+  const X* put(const std::vector<X>& v, hipStream_t s) {
+    return static_cast<const X*>(put_bytes(v.data(), v.size() * sizeof(X), s));
+  }
+
+ private:
+  struct Block {
+    char* h = nullptr;
+    char* d = nullptr;
+    size_t cap = 0;
+  };
+  const void* put_bytes(const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return nullptr;
+    const size_t need = (bytes + 255) & ~size_t(255);
+    while (cur_ < blocks_.size() && off_ + need > blocks_[cur_].cap) {
+      cur_++;
+      off_ = 0;
+    }
+    if (cur_ == blocks_.size()) {
+      Block b;
+      b.cap = need > (size_t(32) << 20) ? need : (size_t(32) << 20);
+      hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&b.h), b.cap, hipHostMallocDefault);
+      if (e != hipSuccess) throw Error(MXG_ERR_NOMEM, std::string("pinned staging allocation failed: ") + hipGetErrorString(e));
+      e = hipMalloc(reinterpret_cast<void**>(&b.d), b.cap);
+      if (e != hipSuccess) {
+        (void)hipHostFree(b.h);
+        throw Error(MXG_ERR_NOMEM, std::string("staging allocation failed: ") + hipGetErrorString(e));
+      }
+      blocks_.push_back(b);
+      off_ = 0;
+    }
+    Block& b = blocks_[cur_];
+    std::memcpy(b.h + off_, src, bytes);
+    MXG_HIP(hipMemcpyAsync(b.d + off_, b.h + off_, bytes, hipMemcpyHostToDevice, s));
+    const void* p = b.d + off_;
+    off_ += need;
+    return p;
+  }
+  std::vector<Block> blocks_;
+  size_t cur_ = 0, off_ = 0;
+};
+
 inline unsigned ceil_div(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
 // ---- device helpers ----
@@ -174,6 +240,8 @@ struct mxg_ctx_s {
     double* hpin = nullptr;  // pinned host copy of one Arnoldi step
     size_t hpin_n = 0;
   } gmres;
+  // descriptor staging of the factorizations on the context stream
+  mxg::Stage stage;
 };
 
 namespace mxg {

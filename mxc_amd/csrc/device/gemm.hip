@@ -265,7 +265,8 @@ __global__ void __launch_bounds__(256) k_bgemm_bf16(const GemmTask<float>* __res
 }  // namespace
 
 template <class T>
-void gemm_batched(const std::vector<GemmTask<T>>& tasks, const std::vector<GemmSeg<T>>& segs, hipStream_t s, bool bf16) {
+void gemm_batched(const std::vector<GemmTask<T>>& tasks, const std::vector<GemmSeg<T>>& segs, hipStream_t s, bool bf16,
+                  Stage* st) {
   if (tasks.empty()) return;
   std::vector<long long> prefix(tasks.size() + 1, 0);
   for (size_t t = 0; t < tasks.size(); t++) {
@@ -280,25 +281,38 @@ void gemm_batched(const std::vector<GemmTask<T>>& tasks, const std::vector<GemmS
   DevBuf<GemmTask<T>> dt;
   DevBuf<GemmSeg<T>> ds;
   DevBuf<long long> dp;
-  dt.upload(tasks, s);
-  ds.upload(segs, s);
-  dp.upload(prefix, s);
+  const GemmTask<T>* pt;
+  const GemmSeg<T>* ps;
+  const long long* pp;
+  if (st) {
+    pt = st->put(tasks, s);
+    ps = st->put(segs, s);
+    pp = st->put(prefix, s);
+  } else {
+    dt.upload(tasks, s);
+    ds.upload(segs, s);
+    dp.upload(prefix, s);
+    pt = dt.p;
+    ps = ds.p;
+    pp = dp.p;
+  }
   if constexpr (std::is_same<T, float>::value) {
     if (bf16)
-      hipLaunchKernelGGL(k_bgemm_bf16, dim3((unsigned)total), dim3(256), 0, s, dt.p, ds.p, dp.p, (int)tasks.size());
+      hipLaunchKernelGGL(k_bgemm_bf16, dim3((unsigned)total), dim3(256), 0, s, pt, ps, pp, (int)tasks.size());
     else
-      hipLaunchKernelGGL(k_bgemm<T>, dim3((unsigned)total), dim3(256), 0, s, dt.p, ds.p, dp.p, (int)tasks.size());
+      hipLaunchKernelGGL(k_bgemm<T>, dim3((unsigned)total), dim3(256), 0, s, pt, ps, pp, (int)tasks.size());
   } else {
     MXG_REQUIRE(!bf16, "bfloat16 inputs are for FP32 products");
-    hipLaunchKernelGGL(k_bgemm<T>, dim3((unsigned)total), dim3(256), 0, s, dt.p, ds.p, dp.p, (int)tasks.size());
+    hipLaunchKernelGGL(k_bgemm<T>, dim3((unsigned)total), dim3(256), 0, s, pt, ps, pp, (int)tasks.size());
   }
   MXG_HIP(hipGetLastError());
-  MXG_HIP(hipStreamSynchronize(s));  // descriptor arrays are freed on return
+  if (!st) MXG_HIP(hipStreamSynchronize(s));  // descriptor arrays are freed on return
 }
 
 template void gemm_batched<double>(const std::vector<GemmTask<double>>&, const std::vector<GemmSeg<double>>&, hipStream_t,
-                                   bool);
-template void gemm_batched<float>(const std::vector<GemmTask<float>>&, const std::vector<GemmSeg<float>>&, hipStream_t, bool);
+                                   bool, Stage*);
+template void gemm_batched<float>(const std::vector<GemmTask<float>>&, const std::vector<GemmSeg<float>>&, hipStream_t, bool,
+                                  Stage*);
 
 }  // namespace mxg
 

@@ -152,7 +152,7 @@ std::vector<T> to_vec(const long long* p, long long n) {
 }  // namespace
 
 template <class T>
-void copy2d_run(const std::vector<Copy2d<T>>& all, hipStream_t s) {
+void copy2d_run(const std::vector<Copy2d<T>>& all, hipStream_t s, Stage* st) {
   std::vector<Copy2d<T>> tasks;
   std::vector<long long> prefix{0};
   for (const Copy2d<T>& c : all) {
@@ -165,16 +165,25 @@ void copy2d_run(const std::vector<Copy2d<T>>& all, hipStream_t s) {
   MXG_REQUIRE(prefix.back() < (1ll << 31), "copy grid too large");
   DevBuf<Copy2d<T>> dt;
   DevBuf<long long> dp;
-  dt.upload(tasks, s);
-  dp.upload(prefix, s);
-  hipLaunchKernelGGL(k_copy2d<T>, dim3((unsigned)prefix.back()), dim3(256), 0, s, dt.p, dp.p, (int)tasks.size());
+  const Copy2d<T>* pt;
+  const long long* pp;
+  if (st) {
+    pt = st->put(tasks, s);
+    pp = st->put(prefix, s);
+  } else {
+    dt.upload(tasks, s);
+    dp.upload(prefix, s);
+    pt = dt.p;
+    pp = dp.p;
+  }
+  hipLaunchKernelGGL(k_copy2d<T>, dim3((unsigned)prefix.back()), dim3(256), 0, s, pt, pp, (int)tasks.size());
   MXG_HIP(hipGetLastError());
-  MXG_HIP(hipStreamSynchronize(s));
+  if (!st) MXG_HIP(hipStreamSynchronize(s));
 }
 
-template void copy2d_run<double>(const std::vector<Copy2d<double>>&, hipStream_t);
-template void copy2d_run<float>(const std::vector<Copy2d<float>>&, hipStream_t);
-template void copy2d_run<int>(const std::vector<Copy2d<int>>&, hipStream_t);
+template void copy2d_run<double>(const std::vector<Copy2d<double>>&, hipStream_t, Stage*);
+template void copy2d_run<float>(const std::vector<Copy2d<float>>&, hipStream_t, Stage*);
+template void copy2d_run<int>(const std::vector<Copy2d<int>>&, hipStream_t, Stage*);
 
 template <class T>
 void copy2d_batched(const std::vector<const T*>& src, const std::vector<T*>& dst, const std::vector<int>& m,

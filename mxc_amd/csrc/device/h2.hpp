@@ -186,9 +186,9 @@ struct Copy2d {
   int m, n, lds, ldd;
   int trans, pad;
 };
-// every copy of the list in one launch
+// every copy of the list in one launch (staged as gemm_batched's)
 template <class T>
-void copy2d_run(const std::vector<Copy2d<T>>& tasks, hipStream_t s);
+void copy2d_run(const std::vector<Copy2d<T>>& tasks, hipStream_t s, Stage* st = nullptr);
 // launch helpers used by ulv.hip / gmres.hip
 template <class T>
 void copy2d_batched(const std::vector<const T*>& src, const std::vector<T*>& dst, const std::vector<int>& m,

@@ -40,10 +40,12 @@ struct GemmTask {
   double alpha, beta;
 };
 // bf16: FP32 tasks whose inputs are rounded to bfloat16 and multiplied on the
-// bfloat16 matrix cores with FP32 accumulation
+// bfloat16 matrix cores with FP32 accumulation. With a stage, the descriptor
+// arrays are staged there and the call returns without waiting for the
+// launch; without one it waits, and the caller may free what it passed.
 template <class T>
 void gemm_batched(const std::vector<GemmTask<T>>& tasks, const std::vector<GemmSeg<T>>& segs, hipStream_t s,
-                  bool bf16 = false);
+                  bool bf16 = false, Stage* st = nullptr);
 
 // ---- dense factorizations, one workgroup per matrix (lapack.hip) ----
 // Householder QR of an m x n column-major matrix in place (R on/above the
@@ -95,7 +97,7 @@ struct LuTask {
   int n, lda;
 };
 template <class T>
-void lu_inverse_batched(const std::vector<LuTask<T>>& tasks, hipStream_t s);
+void lu_inverse_batched(const std::vector<LuTask<T>>& tasks, hipStream_t s, Stage* st = nullptr);
 
 // ---- Rademacher sketch of far-field blocks (sketch.hip) ----
 // Y (M x M, ld M) = Omega G, G(t, j) = K(|body(rows[t]) - body(cols[j])|),

@@ -487,14 +487,20 @@ __global__ void __launch_bounds__(512) k_lu_inverse_reg(const LuTask<T>* __restr
 }
 
 template <class Task, class Kern>
-void launch_tasks(const std::vector<Task>& tasks, Kern kern, hipStream_t s) {
+void launch_tasks(const std::vector<Task>& tasks, Kern kern, hipStream_t s, Stage* st = nullptr) {
   if (tasks.empty()) return;
   MXG_REQUIRE(tasks.size() < (1u << 31), "too many tasks");
   DevBuf<Task> dt;
-  dt.upload(tasks, s);
-  hipLaunchKernelGGL(kern, dim3((unsigned)tasks.size()), dim3(256), 0, s, dt.p);
+  const Task* pt;
+  if (st) {
+    pt = st->put(tasks, s);
+  } else {
+    dt.upload(tasks, s);
+    pt = dt.p;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)tasks.size()), dim3(256), 0, s, pt);
   MXG_HIP(hipGetLastError());
-  MXG_HIP(hipStreamSynchronize(s));
+  if (!st) MXG_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace
@@ -516,18 +522,24 @@ constexpr int lu_max_rows() {
 }
 
 template <class Task, class Kern>
-void launch_tasks_512(const std::vector<Task>& tasks, Kern kern, hipStream_t s) {
+void launch_tasks_512(const std::vector<Task>& tasks, Kern kern, hipStream_t s, Stage* st = nullptr) {
   if (tasks.empty()) return;
   MXG_REQUIRE(tasks.size() < (1u << 31), "too many tasks");
   DevBuf<Task> dt;
-  dt.upload(tasks, s);
-  hipLaunchKernelGGL(kern, dim3((unsigned)tasks.size()), dim3(512), 0, s, dt.p);
+  const Task* pt;
+  if (st) {
+    pt = st->put(tasks, s);
+  } else {
+    dt.upload(tasks, s);
+    pt = dt.p;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)tasks.size()), dim3(512), 0, s, pt);
   MXG_HIP(hipGetLastError());
-  MXG_HIP(hipStreamSynchronize(s));
+  if (!st) MXG_HIP(hipStreamSynchronize(s));
 }
 
 template <class T>
-void lu_inverse_batched(const std::vector<LuTask<T>>& tasks, hipStream_t s) {
+void lu_inverse_batched(const std::vector<LuTask<T>>& tasks, hipStream_t s, Stage* st) {
   for (const LuTask<T>& t : tasks) MXG_REQUIRE(t.n >= 0 && t.n <= 1024, "LU block larger than 1024");
   // by size: the register kernel for each tile shape in use, the
   // global-memory kernel above lu_max_rows
@@ -539,14 +551,14 @@ void lu_inverse_batched(const std::vector<LuTask<T>>& tasks, hipStream_t s) {
     else if (t.n <= lu_max_rows<T>()) by[3].push_back(t);
     else big.push_back(t);
   }
-  launch_tasks_512(by[0], k_lu_inverse_reg<T, 4, 2>, s);
-  launch_tasks_512(by[1], k_lu_inverse_reg<T, 8, 4>, s);
-  launch_tasks_512(by[2], k_lu_inverse_reg<T, 12, 6>, s);
-  if constexpr (lu_max_rows<T>() >= 256) launch_tasks_512(by[3], k_lu_inverse_reg<T, 16, 8>, s);
-  launch_tasks(big, k_lu_inverse<T>, s);
+  launch_tasks_512(by[0], k_lu_inverse_reg<T, 4, 2>, s, st);
+  launch_tasks_512(by[1], k_lu_inverse_reg<T, 8, 4>, s, st);
+  launch_tasks_512(by[2], k_lu_inverse_reg<T, 12, 6>, s, st);
+  if constexpr (lu_max_rows<T>() >= 256) launch_tasks_512(by[3], k_lu_inverse_reg<T, 16, 8>, s, st);
+  launch_tasks(big, k_lu_inverse<T>, s, st);
 }
 
-template void lu_inverse_batched<double>(const std::vector<LuTask<double>>&, hipStream_t);
-template void lu_inverse_batched<float>(const std::vector<LuTask<float>>&, hipStream_t);
+template void lu_inverse_batched<double>(const std::vector<LuTask<double>>&, hipStream_t, Stage*);
+template void lu_inverse_batched<float>(const std::vector<LuTask<float>>&, hipStream_t, Stage*);
 
 }  // namespace mxg

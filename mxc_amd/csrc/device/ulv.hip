@@ -63,7 +63,7 @@ struct GemmList {
     g.insert(g.end(), o.g.begin(), o.g.end());
   }
   bool bf16 = false;  // FP32 products with bfloat16 inputs
-  void run(hipStream_t s) const { gemm_batched<T>(t, g, s, bf16); }
+  void run(hipStream_t s, Stage* st) const { gemm_batched<T>(t, g, s, bf16, st); }
 };
 
 // The launches of one level's factorization in their order. Plans of the
@@ -108,21 +108,24 @@ struct LevelPlan {
     sch2.bf16 = sch2.bf16 || o.sch2.bf16;
     for (auto& b : o.scratch) scratch.push_back(std::move(b));
   }
-  void run(hipStream_t s) const {
-    rot1.run(s);
-    rot2.run(s);
-    lu_inverse_batched<T>(lu, s);
-    copy2d_run<T>(vtop, s);
-    vbot.run(s);
-    rs.run(s);
-    ss.run(s);
-    copy2d_run<T>(rscopy, s);
+  // the launches, in order, with their descriptors staged in st: they have
+  // not necessarily run when this returns, so the plan (its scratch) must be
+  // kept until the stream has been waited for
+  void run(hipStream_t s, Stage* st) const {
+    rot1.run(s, st);
+    rot2.run(s, st);
+    lu_inverse_batched<T>(lu, s, st);
+    copy2d_run<T>(vtop, s, st);
+    vbot.run(s, st);
+    rs.run(s, st);
+    ss.run(s, st);
+    copy2d_run<T>(rscopy, s, st);
     for (size_t c = 0; c < offa.size(); c++) {
-      offa[c].run(s);
-      offb[c].run(s);
+      offa[c].run(s, st);
+      offb[c].run(s, st);
     }
-    sch1.run(s);
-    sch2.run(s);
+    sch1.run(s, st);
+    sch2.run(s, st);
   }
 };
 
@@ -136,25 +139,39 @@ struct UlvImpl;
 // depth, box sizes, ranks and near lists; every matrix gets the same results
 // as when factorized alone (the tasks are the same, only their launch is
 // shared).
+//
+// No launch waits for the previous one on the host: the descriptor arrays go
+// through the context's stage and every level's plan (its scratch) is kept
+// until the end, so the host plans level t + 1 while the GPU runs level t.
 template <class T, class S>
 void factorize_batch(const std::vector<UlvImpl<T, S>*>& fs, hipStream_t s) {
   long long depth = 0;
   for (auto* f : fs) depth = std::max(depth, f->h->L);
-  for (long long t = 0; t <= depth; t++) {
-    LevelPlan<T> P;
-    std::vector<Copy2d<T>> next;
-    for (auto* f : fs) {
-      const long long l = f->h->L - t;
-      if (l < 0) continue;
-      LevelPlan<T> q;
-      f->plan_level(l, q);
-      P.merge(std::move(q));
-      if (l > 0) f->plan_copy_next(l, next);
+  Stage& st = fs.front()->h->ctx->stage;
+  st.rewind(s);
+  std::vector<LevelPlan<T>> plans(depth + 1);
+  try {
+    for (long long t = 0; t <= depth; t++) {
+      LevelPlan<T>& P = plans[t];
+      std::vector<Copy2d<T>> next;
+      for (auto* f : fs) {
+        const long long l = f->h->L - t;
+        if (l < 0) continue;
+        LevelPlan<T> q;
+        f->plan_level(l, q);
+        P.merge(std::move(q));
+        if (l > 0) f->plan_copy_next(l, next);
+      }
+      P.run(s, &st);
+      copy2d_run<T>(next, s, &st);
     }
-    P.run(s);
-    copy2d_run<T>(next, s);
+    for (auto* f : fs) f->finish();
+  } catch (...) {
+    // the launches already issued read the plans' scratch: wait before it goes
+    (void)hipStreamSynchronize(s);
+    throw;
   }
-  for (auto* f : fs) f->finish();
+  MXG_HIP(hipStreamSynchronize(s));
 }
 
 struct UlvBase {
