@@ -156,6 +156,31 @@ def test_factorization_substitution_matches_oracle(setup, preconditioners):
     M.factorize(N.MXG_FP64)
 
 
+def test_refactorization_is_reproducible(setup, preconditioners):
+    """Factorizations of one matrix in a row, in alternating precisions, reuse
+    the context's staged task lists; each gives the same substitution bit for
+    bit as the first one of its precision."""
+    M, _ = preconditioners
+    ctx = setup["ctx"]
+
+    def substitute():
+        d = ctx.array(setup["bref"])
+        M.precondition(d)
+        z = d.download()
+        d.free()
+        return z
+
+    first = {}
+    for prec in (N.MXG_FP64, N.MXG_BF16, N.MXG_FP64, N.MXG_FP32, N.MXG_BF16, N.MXG_FP32):
+        assert not M.factorize(prec).any()
+        z = substitute()
+        if prec in first:
+            assert np.array_equal(z, first[prec]), prec
+        else:
+            first[prec] = z
+    M.factorize(N.MXG_FP64)
+
+
 @pytest.mark.parametrize("precision,tol", [(N.MXG_FP64, 1e-11), (N.MXG_FP32, 1e-4)])
 def test_hss_ulv_is_exact(setup, precision, tol):
     """theta = 0 makes the preconditioner an HSS matrix, for which the ULV
