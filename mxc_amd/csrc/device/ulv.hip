@@ -18,7 +18,10 @@
 //   7  A_ij[:s, :s] -> the parent's near block (factorizeCopyNext,
 //      h2matrix.cpp:427-446)
 // Every step is one batched launch over the whole level; mxg_ulv_create_batch
-// shares each launch among the same level steps of several matrices.
+// shares each launch among the same level steps of several matrices. The
+// launches go out with no wait on the host between them: their task lists
+// are staged through pinned memory kept on the context, and every level's
+// scratch lives until the end of the factorization.
 //
 // Substitution (h2matrix.cpp:448-537): forward L -> 0, x_i = V_i y_i,
 // z_i = x_i[:s] - sum_j A_ij.sr x_j[s_j:]; backward 0 -> L,
