@@ -99,6 +99,61 @@ struct DevBuf {
   }
 };
 
+// Device scratch handed out in order and taken back all at once by rewind(),
+// which first waits for the stream. Its blocks stay allocated for later use,
+// so a sequence of requests that repeats (a matrix factorized again, or in a
+// lower precision) calls no hipMalloc: allocations of several GB have taken
+// up to seconds each on this runtime when large buffers had been freed
+// shortly before. Contents are not cleared between uses.
+class Arena {
+ public:
+  Arena() = default;
+  Arena(const Arena&) = delete;
+  Arena& operator=(const Arena&) = delete;
+  ~Arena() {
+    for (Block& b : blocks_) (void)hipFree(b.p);
+  }
+  void rewind(hipStream_t s) {
+    MXG_HIP(hipStreamSynchronize(s));
+    cur_ = 0;
+    off_ = 0;
+  }
+  void* take(size_t bytes) {
+    const size_t need = ((bytes ? bytes : 1) + 255) & ~size_t(255);
+    while (cur_ < blocks_.size() && off_ + need > blocks_[cur_].cap) {
+      cur_++;
+      off_ = 0;
+    }
+    if (cur_ == blocks_.size()) {
+      constexpr size_t G = size_t(2) << 20;
+      Block b;
+      b.cap = (need + G - 1) / G * G;
+      hipError_t e = hipMalloc(reinterpret_cast<void**>(&b.p), b.cap);
+      if (e != hipSuccess)
+        throw Error(MXG_ERR_NOMEM, "hipMalloc of " + std::to_string(b.cap) + " bytes of scratch failed: " + hipGetErrorString(e));
+      blocks_.push_back(b);
+      off_ = 0;
+    }
+    char* p = blocks_[cur_].p + off_;
+    off_ += need;
+    return p;
+  }
+  // device bytes held
+  size_t held() const {
+    size_t t = 0;
+    for (const Block& b : blocks_) t += b.cap;
+    return t;
+  }
+
+ private:
+  struct Block {
+    char* p = nullptr;
+    size_t cap = 0;
+  };
+  std::vector<Block> blocks_;
+  size_t cur_ = 0, off_ = 0;
+};
+
 // Descriptor arrays (task lists) of a sequence of launches on one stream. An
 // array is copied into pinned host memory and from there, by an asynchronous
 // copy on the stream, into device memory; both stay valid until rewind(),
@@ -240,8 +295,10 @@ struct mxg_ctx_s {
     double* hpin = nullptr;  // pinned host copy of one Arnoldi step
     size_t hpin_n = 0;
   } gmres;
-  // descriptor staging of the factorizations on the context stream
+  // descriptor staging and per-level scratch of the factorizations on the
+  // context stream
   mxg::Stage stage;
+  mxg::Arena fscratch;
 };
 
 namespace mxg {

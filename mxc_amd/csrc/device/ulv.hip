@@ -21,7 +21,8 @@
 // shares each launch among the same level steps of several matrices. The
 // launches go out with no wait on the host between them: their task lists
 // are staged through pinned memory kept on the context, and every level's
-// scratch lives until the end of the factorization.
+// scratch comes from an arena kept on the context, valid until the next
+// factorization there (which reuses it instead of allocating).
 //
 // Substitution (h2matrix.cpp:448-537): forward L -> 0, x_i = V_i y_i,
 // z_i = x_i[:s] - sum_j A_ij.sr x_j[s_j:]; backward 0 -> L,
@@ -82,11 +83,10 @@ struct LevelPlan {
   std::vector<Copy2d<T>> rscopy;          // 4
   std::vector<GemmList<T>> offa, offb;    // 5, per chunk of pairs
   GemmList<T> sch1, sch2;                 // 6
-  std::vector<std::unique_ptr<DevBuf<T>>> scratch;  // held until the launches ran
+  Arena* arena = nullptr;  // the scratch: valid until the arena is rewound
   T* alloc(size_t n) {
-    scratch.emplace_back(new DevBuf<T>());
-    scratch.back()->alloc(std::max<size_t>(n, 1));
-    return scratch.back()->p;
+    MXG_REQUIRE(arena, "level plan without scratch arena");
+    return static_cast<T*>(arena->take(std::max<size_t>(n, 1) * sizeof(T)));
   }
   void merge(LevelPlan&& o) {
     rot1.append(o.rot1);
@@ -109,11 +109,9 @@ struct LevelPlan {
     sch2.append(o.sch2);
     sch1.bf16 = sch1.bf16 || o.sch1.bf16;
     sch2.bf16 = sch2.bf16 || o.sch2.bf16;
-    for (auto& b : o.scratch) scratch.push_back(std::move(b));
   }
   // the launches, in order, with their descriptors staged in st: they have
-  // not necessarily run when this returns, so the plan (its scratch) must be
-  // kept until the stream has been waited for
+  // not necessarily run when this returns
   void run(hipStream_t s, Stage* st) const {
     rot1.run(s, st);
     rot2.run(s, st);
@@ -144,14 +142,17 @@ struct UlvImpl;
 // shared).
 //
 // No launch waits for the previous one on the host: the descriptor arrays go
-// through the context's stage and every level's plan (its scratch) is kept
-// until the end, so the host plans level t + 1 while the GPU runs level t.
+// through the context's stage, every level's scratch comes from the
+// context's arena and stays valid until the end, and the plans are kept, so
+// the host plans level t + 1 while the GPU runs level t.
 template <class T, class S>
 void factorize_batch(const std::vector<UlvImpl<T, S>*>& fs, hipStream_t s) {
   long long depth = 0;
   for (auto* f : fs) depth = std::max(depth, f->h->L);
   Stage& st = fs.front()->h->ctx->stage;
   st.rewind(s);
+  Arena& arena = fs.front()->h->ctx->fscratch;
+  arena.rewind(s);
   std::vector<LevelPlan<T>> plans(depth + 1);
   try {
     for (long long t = 0; t <= depth; t++) {
@@ -161,6 +162,7 @@ void factorize_batch(const std::vector<UlvImpl<T, S>*>& fs, hipStream_t s) {
         const long long l = f->h->L - t;
         if (l < 0) continue;
         LevelPlan<T> q;
+        q.arena = &arena;
         f->plan_level(l, q);
         P.merge(std::move(q));
         if (l > 0) f->plan_copy_next(l, next);
@@ -170,7 +172,8 @@ void factorize_batch(const std::vector<UlvImpl<T, S>*>& fs, hipStream_t s) {
     }
     for (auto* f : fs) f->finish();
   } catch (...) {
-    // the launches already issued read the plans' scratch: wait before it goes
+    // the launches already issued may still be running on the factors, which
+    // the caller frees on an error: wait for them first
     (void)hipStreamSynchronize(s);
     throw;
   }
@@ -335,7 +338,7 @@ struct UlvImpl : UlvBase {
   }
 
   // The launches of level l's factorization, appended to P (steps 1-6 of the
-  // header; P owns the scratch they use).
+  // header; their scratch comes from P's arena).
   void plan_level(long long l, LevelPlan<T>& P) {
     Level& H = h->lv[l];
     Lv& F = lv[l];
